@@ -1,0 +1,213 @@
+/*
+ * gpu_tracking.h -- C ABI of the MI355X (gfx950) locate library
+ * (libgpu_tracking.so).
+ *
+ * The ABI replaces the per-frame locate path of the reference crate
+ * `gpu_tracking` (gpu-tracking/src/execute_gpu.rs, gpu-tracking/src/lib.rs).
+ * Every entry point names the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Plain C types only.  Nothing unwinds across the ABI: every call returns a
+ *     gt_status; gt_last_error() gives the calling thread's last message.
+ *   - A gt_context owns one HIP device, its stream, its cached device
+ *     workspaces and compiled-shape caches.  A context is not thread-safe:
+ *     use one context per host thread.
+ *   - Frames are row-major H x W.  Host pointers are read during the call;
+ *     device pointers (GT_FRAMES_ON_DEVICE) must stay valid until it returns.
+ *   - Results are float32 rows in the reference's column order
+ *     (execute_gpu.rs:197-226), one row per located feature.
+ */
+#ifndef GPU_TRACKING_H
+#define GPU_TRACKING_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_ABI_VERSION 1
+
+typedef enum gt_status {
+    GT_OK = 0,
+    GT_E_INVALID = 1,          /* bad argument or parameter combination */
+    GT_E_NOT_IMPLEMENTED = 2,  /* parameter the reference also rejects */
+    GT_E_NO_DEVICE = 3,        /* no HIP device / runtime unavailable */
+    GT_E_HIP = 4,              /* a HIP runtime call failed */
+    GT_E_OOM = 5,              /* host or device allocation failed */
+    GT_E_INTERNAL = 6
+} gt_status;
+
+/* Mirror of `TrackingParams` (execute_gpu.rs:16-38), field for field and in
+ * the same order.  An `Option<T>` field of the reference becomes
+ * `has_<name>` + value.  The six fields that lib.rs:73-74 rejects when passed
+ * from Python (maxsize, threshold, invert, percentile, topn, preprocess)
+ * carry a `has_<name>` flag meaning "explicitly set"; setting any of them
+ * gives GT_E_NOT_IMPLEMENTED, except `percentile`, which this library
+ * implements (exact per-frame quantile of the bandpassed frame, candidate
+ * kept when its value is strictly above it). */
+typedef struct gt_params {
+    uint32_t diameter;
+    float minmass;
+    uint8_t has_maxsize;
+    float maxsize;
+    uint32_t separation;
+    float noise_size;
+    uint32_t smoothing_size;
+    uint8_t has_threshold;
+    float threshold;
+    uint8_t has_invert;
+    uint8_t invert;
+    uint8_t has_percentile;
+    float percentile;
+    uint8_t has_topn;
+    uint32_t topn;
+    uint8_t has_preprocess;
+    uint8_t preprocess;
+    uint32_t max_iterations;
+    uint8_t characterize;
+    uint8_t filter_close;
+    uint8_t has_search_range;
+    float search_range;
+    uint8_t has_memory;
+    uint64_t memory;
+    uint8_t cpu_processed;
+    uint8_t has_sig_radius;
+    float sig_radius;
+    uint8_t has_bg_radius;
+    float bg_radius;
+    uint8_t has_gap_radius;
+    float gap_radius;
+} gt_params;
+
+/* One frame of an uneven batch (replaces one item of the frame iterator fed
+ * to execute_gpu, execute_gpu.rs:372). */
+typedef struct gt_frame_desc {
+    const void* data;       /* H x W pixels, dtype given by the call */
+    uint32_t h, w;
+    int64_t frame_index;    /* value written to the `frame` column */
+} gt_frame_desc;
+
+typedef struct gt_context gt_context;
+typedef struct gt_result gt_result;
+
+/* Flags of the locate calls. */
+#define GT_FRAMES_ON_DEVICE  0x1u  /* frame pointers are HIP device pointers */
+#define GT_PLAIN             0x2u  /* per-frame pipeline: launch + readback per frame */
+#define GT_RESULT_ON_DEVICE  0x4u  /* leave rows in device memory (see gt_result_device_data) */
+
+/* Per-call device timing of the last locate call on a context, measured with
+ * HIP events on the context's stream. */
+typedef struct gt_timing {
+    double device_ms;       /* first kernel start .. last kernel end */
+    double total_ms;        /* host wall time of the call */
+    uint64_t frames;
+    uint64_t pixels;
+    uint64_t candidates;    /* local maxima that entered refinement */
+    uint64_t rows;          /* rows returned */
+    uint64_t launches;      /* kernel launches issued */
+} gt_timing;
+
+/* --- parameters (no device needed) ------------------------------------- */
+
+/* Python defaults of batch_rust (lib.rs:77-91): minmass 0, separation
+ * diameter+1, noise_size 1, smoothing_size diameter, max_iterations 10,
+ * characterize false, filter_close true, cpu_processed false; all optional
+ * fields unset. */
+gt_status gt_params_default(gt_params* p, uint32_t diameter);
+
+/* Rust `Default` of TrackingParams (execute_gpu.rs:40-65). */
+gt_status gt_params_rust_default(gt_params* p);
+
+/* Checks a parameter set for frames of h x w (h = w = 0: shape-free checks).
+ * Returns GT_E_NOT_IMPLEMENTED for what lib.rs:73-74 rejects. */
+gt_status gt_params_validate(const gt_params* p, uint32_t h, uint32_t w);
+
+/* column_names (execute_gpu.rs:197-226): names and dtypes ("int"/"float")
+ * of the result columns for this parameter set.  Pointers are static. */
+gt_status gt_column_names(const gt_params* p, const char** names,
+                          const char** dtypes, uint32_t cap, uint32_t* ncols);
+
+const char* gt_last_error(void);
+const char* gt_version(void);
+
+/* Number of visible HIP devices (0 when there is none; never fails). */
+int gt_device_count(void);
+
+/* --- context ----------------------------------------------------------- */
+
+/* Replaces gpu_setup::setup_state (gpu_setup.rs:191), which the reference
+ * runs per call; here the device state persists across calls. */
+gt_status gt_context_create(int device, gt_context** out);
+gt_status gt_context_destroy(gt_context* ctx);
+gt_status gt_context_timing(const gt_context* ctx, gt_timing* out);
+/* Frames per launch of the batched path (0 = automatic). */
+gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch);
+
+/* --- locate ------------------------------------------------------------- */
+
+/* execute_ndarray (execute_gpu.rs:496) / batch_rust (lib.rs:46) for a
+ * C-contiguous (n_frames, h, w) uint16 movie.  uint16 -> float32 is exact,
+ * so results equal those for the same frames as float32. */
+gt_status gt_locate_u16(gt_context* ctx, const uint16_t* frames,
+                        uint64_t n_frames, uint32_t h, uint32_t w,
+                        const gt_params* p, uint32_t flags, gt_result** out);
+
+/* Same for uint8 frames (the reference converts u8 pages to f32,
+ * decoderiter.rs:31-35). */
+gt_status gt_locate_u8(gt_context* ctx, const uint8_t* frames,
+                       uint64_t n_frames, uint32_t h, uint32_t w,
+                       const gt_params* p, uint32_t flags, gt_result** out);
+
+/* Same for float32 frames, the reference's own input type (lib.rs:50).
+ * Pixel values must be finite. */
+gt_status gt_locate_f32(gt_context* ctx, const float* frames,
+                        uint64_t n_frames, uint32_t h, uint32_t w,
+                        const gt_params* p, uint32_t flags, gt_result** out);
+
+/* Frames of different shapes in one batched launch sequence; dtype 0 =
+ * uint16, 1 = float32, 2 = uint8.  Rows are grouped by descriptor order. */
+gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames,
+                          uint64_t n_frames, int dtype, const gt_params* p,
+                          uint32_t flags, gt_result** out);
+
+/* --- results ------------------------------------------------------------ */
+
+uint64_t gt_result_rows(const gt_result* r);
+uint32_t gt_result_ncols(const gt_result* r);
+/* Host rows, row-major n_rows x ncols float32 (NULL with GT_RESULT_ON_DEVICE). */
+const float* gt_result_data(const gt_result* r);
+/* Candidate (start) pixel index y*w + x of each row, host int64 (NULL with
+ * GT_RESULT_ON_DEVICE).  Not a reference column: a key for comparisons. */
+const int64_t* gt_result_candidates(const gt_result* r);
+/* Device rows (GT_RESULT_ON_DEVICE only): valid until the next locate call
+ * on the same context or gt_result_free. */
+const float* gt_result_device_data(const gt_result* r);
+gt_status gt_result_columns(const gt_result* r, const char** names,
+                            const char** dtypes, uint32_t cap, uint32_t* ncols);
+void gt_result_free(gt_result* r);
+
+/* --- synthetic movies (device side, for tests and benchmarks) ------------ */
+
+/* Fills n_frames x h x w uint16 device frames with
+ * min(Poisson(bg[f] + sum_i amp_i exp(-d^2 / (2 sigma_i^2))), 65535), spots
+ * truncated at 5 sigma, Poisson sampled with Hoermann's PTRS (lambda >= 10)
+ * or sequential inversion (lambda < 10) from Philox-4x32-10 streams keyed by
+ * (seed, frame).  spots: host array of n_spots x 4 floats (y, x, amp, sigma)
+ * grouped by frame with frame f's spots at [spot_off[f], spot_off[f+1]). */
+gt_status gt_synth_u16(gt_context* ctx, uint16_t* d_frames, uint64_t n_frames,
+                       uint32_t h, uint32_t w, const float* spots,
+                       const uint64_t* spot_off, const float* bg, uint64_t seed);
+
+/* Device memory helpers for callers without another allocator. */
+gt_status gt_device_alloc(gt_context* ctx, uint64_t bytes, void** out);
+gt_status gt_device_free(gt_context* ctx, void* p);
+gt_status gt_memcpy_h2d(gt_context* ctx, void* dst, const void* src, uint64_t bytes);
+gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t bytes);
+gt_status gt_synchronize(gt_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: this module is the checker for the HIP locate path.
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+import it; the product package never does.
+
+The oracle restates the reference's locate pipeline in C (see gt_oracle.h for
+the file:line citations and the arithmetic contract).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("diameter", ctypes.c_uint32),
+        ("minmass", ctypes.c_float),
+        ("separation", ctypes.c_uint32),
+        ("noise_size", ctypes.c_float),
+        ("smoothing_size", ctypes.c_uint32),
+        ("max_iterations", ctypes.c_uint32),
+        ("has_percentile", ctypes.c_int32),
+        ("percentile", ctypes.c_float),
+        ("characterize", ctypes.c_int32),
+        ("cpu_processed", ctypes.c_int32),
+        ("literal", ctypes.c_int32),
+    ]
+
+
+class OrcGeom(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_int32),
+        ("R", ctypes.c_int32),
+        ("band", ctypes.c_int32),
+        ("rint", ctypes.c_int32),
+        ("kd", ctypes.c_int32),
+        ("a", ctypes.c_int32),
+        ("r2", ctypes.c_float),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (idempotent)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        fp = ctypes.POINTER(ctypes.c_float)
+        dp = ctypes.POINTER(ctypes.c_double)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.orc_geometry.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcGeom)]
+        L.orc_gauss_weights.argtypes = [ctypes.c_int32, ctypes.c_float, fp, fp]
+        L.orc_bandpass_f32.argtypes = [fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_float, fp]
+        L.orc_bandpass_f64.argtypes = [dp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_double, dp]
+        L.orc_masses_centers_f32.argtypes = [fp, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_uint32, fp, fp]
+        L.orc_is_max_map.argtypes = [fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, u8p]
+        L.orc_percentile_threshold.argtypes = [fp, ctypes.c_size_t, ctypes.c_float]
+        L.orc_percentile_threshold.restype = ctypes.c_double
+        L.orc_locate_frame.argtypes = [fp, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(OrcParams), ctypes.c_int32,
+                                       ctypes.c_int32, dp, i64p, ctypes.c_int64, i64p, dp]
+        L.orc_ncols.argtypes = [ctypes.POINTER(OrcParams)]
+        L.orc_ncols.restype = ctypes.c_int32
+        L.orc_candidates_frame.argtypes = [fp, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(OrcParams), i64p, fp, fp, fp,
+                                           ctypes.c_int64, i64p]
+        L.orc_dedup_bruteforce.argtypes = [fp, fp, fp, i64p, ctypes.c_int64,
+                                           ctypes.c_uint32, u8p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+@dataclass
+class Params:
+    """Locate parameters, Python defaults of lib.rs:77-91."""
+    diameter: int
+    minmass: float = 0.0
+    separation: int | None = None
+    noise_size: float = 1.0
+    smoothing_size: int | None = None
+    max_iterations: int = 10
+    percentile: float | None = None
+    characterize: bool = False
+    cpu_processed: bool = False
+    literal: bool = False
+
+    def c(self) -> OrcParams:
+        return OrcParams(
+            diameter=self.diameter,
+            minmass=self.minmass,
+            separation=self.diameter + 1 if self.separation is None else self.separation,
+            noise_size=self.noise_size,
+            smoothing_size=self.diameter if self.smoothing_size is None else self.smoothing_size,
+            max_iterations=self.max_iterations,
+            has_percentile=0 if self.percentile is None else 1,
+            percentile=0.0 if self.percentile is None else self.percentile,
+            characterize=int(self.characterize),
+            cpu_processed=int(self.cpu_processed),
+            literal=int(self.literal),
+        )
+
+
+def geometry(p: Params) -> OrcGeom:
+    g = OrcGeom()
+    cp = p.c()
+    if lib().orc_geometry(ctypes.byref(cp), ctypes.byref(g)) != 0:
+        raise ValueError("invalid parameters")
+    return g
+
+
+def gauss_weights(k: int, noise_size: float):
+    w = np.zeros(2 * (k // 2) + 1, np.float32)
+    norm = ctypes.c_float()
+    lib().orc_gauss_weights(k, noise_size, _ptr(w, ctypes.c_float), ctypes.byref(norm))
+    return w, np.float32(norm.value)
+
+
+def bandpass(frame: np.ndarray, k: int, noise_size: float = 1.0, f64: bool = False) -> np.ndarray:
+    H, W = frame.shape
+    if f64:
+        x = np.ascontiguousarray(frame, np.float64)
+        out = np.zeros((H, W), np.float64)
+        rc = lib().orc_bandpass_f64(_ptr(x, ctypes.c_double), H, W, k, noise_size,
+                                    _ptr(out, ctypes.c_double))
+    else:
+        x = np.ascontiguousarray(frame, np.float32)
+        out = np.zeros((H, W), np.float32)
+        rc = lib().orc_bandpass_f32(_ptr(x, ctypes.c_float), H, W, k, noise_size,
+                                    _ptr(out, ctypes.c_float))
+    if rc != 0:
+        raise ValueError(f"bandpass rejected input (rc={rc})")
+    return out
+
+
+def masses_centers(P: np.ndarray, diameter: int):
+    H, W = P.shape
+    x = np.ascontiguousarray(P, np.float32)
+    m = np.zeros((H, W), np.float32)
+    c = np.zeros((H, W, 2), np.float32)
+    lib().orc_masses_centers_f32(_ptr(x, ctypes.c_float), H, W, diameter,
+                                 _ptr(m, ctypes.c_float), _ptr(c, ctypes.c_float))
+    return m, c
+
+
+def is_max_map(P: np.ndarray, separation: int) -> np.ndarray:
+    H, W = P.shape
+    x = np.ascontiguousarray(P, np.float32)
+    out = np.zeros((H, W), np.uint8)
+    lib().orc_is_max_map(_ptr(x, ctypes.c_float), H, W, separation, _ptr(out, ctypes.c_uint8))
+    return out.astype(bool)
+
+
+def percentile_threshold(P: np.ndarray, percentile: float) -> float:
+    x = np.ascontiguousarray(P, np.float32).ravel()
+    return lib().orc_percentile_threshold(_ptr(x, ctypes.c_float), x.size, percentile)
+
+
+def ncols(p: Params) -> int:
+    cp = p.c()
+    return lib().orc_ncols(ctypes.byref(cp))
+
+
+def locate_frame(frame: np.ndarray, p: Params, frame_index: int = 0, f64: bool = False):
+    """Rows (float64, column_names order) and candidate pixel index per row,
+    in raster order of the candidate pixel; also the percentile threshold."""
+    H, W = frame.shape
+    x = np.ascontiguousarray(frame, np.float32)
+    cp = p.c()
+    n = ctypes.c_int64()
+    thr = ctypes.c_double()
+    L = lib()
+    nc = L.orc_ncols(ctypes.byref(cp))
+    cap = 4096
+    while True:
+        rows = np.zeros((cap, nc), np.float64)
+        cand = np.zeros(cap, np.int64)
+        rc = L.orc_locate_frame(_ptr(x, ctypes.c_float), H, W, ctypes.byref(cp), frame_index,
+                                int(f64), _ptr(rows, ctypes.c_double),
+                                _ptr(cand, ctypes.c_int64), cap, ctypes.byref(n),
+                                ctypes.byref(thr))
+        if rc == -3:
+            cap = int(n.value)
+            continue
+        if rc != 0:
+            raise ValueError(f"locate rejected input (rc={rc})")
+        k = int(n.value)
+        return rows[:k].copy(), cand[:k].copy(), thr.value
+
+
+def candidates_frame(frame: np.ndarray, p: Params):
+    H, W = frame.shape
+    x = np.ascontiguousarray(frame, np.float32)
+    cp = p.c()
+    n = ctypes.c_int64()
+    cap = 4096
+    while True:
+        cand = np.zeros(cap, np.int64)
+        y = np.zeros(cap, np.float32)
+        xx = np.zeros(cap, np.float32)
+        m = np.zeros(cap, np.float32)
+        rc = lib().orc_candidates_frame(_ptr(x, ctypes.c_float), H, W, ctypes.byref(cp),
+                                        _ptr(cand, ctypes.c_int64), _ptr(y, ctypes.c_float),
+                                        _ptr(xx, ctypes.c_float), _ptr(m, ctypes.c_float),
+                                        cap, ctypes.byref(n))
+        if rc == -3:
+            cap = int(n.value)
+            continue
+        if rc != 0:
+            raise ValueError(f"candidates rejected input (rc={rc})")
+        k = int(n.value)
+        return cand[:k], y[:k], xx[:k], m[:k]
+
+
+def dedup_bruteforce(y, x, mass, idx, separation: int) -> np.ndarray:
+    y = np.ascontiguousarray(y, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    mass = np.ascontiguousarray(mass, np.float32)
+    idx = np.ascontiguousarray(idx, np.int64)
+    keep = np.zeros(len(y), np.uint8)
+    lib().orc_dedup_bruteforce(_ptr(y, ctypes.c_float), _ptr(x, ctypes.c_float),
+                               _ptr(mass, ctypes.c_float), _ptr(idx, ctypes.c_int64),
+                               len(y), separation, _ptr(keep, ctypes.c_uint8))
+    return keep.astype(bool)

@@ -1,0 +1,152 @@
+"""GPU locate vs the CPU oracle on the same frames (bit-exact candidate sets,
+positions within 1e-4 px, mass and characterisation within 1e-5 relative).
+
+Frames come from the device generator (copied back for the oracle), from the
+reference's own grey_lion frame (tests/golden) and from crops of odd sizes
+that exercise partial tiles and frame borders.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tracking_optimizations_amd import _lib, synth
+from tracking_optimizations_amd.gpu_tracking import locate, make_params
+
+pytestmark = pytest.mark.gpu
+
+POS_TOL = 1e-4     # px, on y and x
+REL_TOL = 1e-5     # mass and characterisation columns
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _lib.default_context(0)
+
+
+@pytest.fixture(scope="module")
+def frames512(ctx):
+    return synth.generate_host(ctx, synth.single_512(seed=1, n_frames=3))
+
+
+@pytest.fixture(scope="module")
+def grey_lion():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "grey_lion_u8.npy"))
+
+
+CASES = {
+    "c1": dict(diameter=9, minmass=1000, separation=9, percentile=64),
+    "c2": dict(diameter=11, minmass=0, separation=11, smoothing_size=11, percentile=64),
+    "defaults": dict(diameter=9),
+    "char": dict(diameter=9, separation=9, characterize=True, cpu_processed=True),
+    "even_d": dict(diameter=10, separation=12, smoothing_size=7),
+    "iter1": dict(diameter=9, separation=9, max_iterations=1),
+    "iter0": dict(diameter=9, separation=9, max_iterations=0),
+    "small_sep": dict(diameter=7, separation=5, smoothing_size=9),
+    "wide_smooth": dict(diameter=9, separation=9, smoothing_size=17, noise_size=2.0),
+    "wide_sep": dict(diameter=9, separation=20),
+    "pct0": dict(diameter=9, separation=9, percentile=0),
+    "pct100": dict(diameter=9, separation=9, percentile=100),
+}
+
+
+def _oracle_params(kw):
+    return oracle.Params(
+        diameter=kw["diameter"], minmass=kw.get("minmass", 0.0),
+        separation=kw.get("separation"), noise_size=kw.get("noise_size", 1.0),
+        smoothing_size=kw.get("smoothing_size"), max_iterations=kw.get("max_iterations", 10),
+        percentile=kw.get("percentile"), characterize=kw.get("characterize", False),
+        cpu_processed=kw.get("cpu_processed", False))
+
+
+def _oracle_movie(frames, kw):
+    rows, cands = [], []
+    for i, f in enumerate(frames):
+        r, c, _ = oracle.locate_frame(f.astype(np.float32), _oracle_params(kw), frame_index=i)
+        rows.append(r)
+        cands.append(c)
+    ncol = oracle.ncols(_oracle_params(kw))
+    return (np.concatenate(rows) if rows else np.zeros((0, ncol)),
+            np.concatenate(cands) if cands else np.zeros(0, np.int64))
+
+
+def _check(rows, cand, ref_rows, ref_cand, label):
+    assert rows.shape[1] == ref_rows.shape[1], label
+    o = np.lexsort((cand, rows[:, 0]))
+    r = np.lexsort((ref_cand, ref_rows[:, 0]))
+    rows, cand = rows[o].astype(np.float64), cand[o]
+    ref_rows, ref_cand = ref_rows[r], ref_cand[r]
+    assert len(cand) == len(ref_cand), f"{label}: {len(cand)} rows vs oracle {len(ref_cand)}"
+    assert np.array_equal(cand, ref_cand), f"{label}: candidate pixel sets differ"
+    assert np.array_equal(rows[:, 0], ref_rows[:, 0]), f"{label}: frame column differs"
+    if len(cand) == 0:
+        return
+    dpos = np.abs(rows[:, 2:4] - ref_rows[:, 2:4]).max()
+    assert dpos <= POS_TOL, f"{label}: position error {dpos}"
+    rel = np.abs(rows[:, 1] - ref_rows[:, 1]) / np.maximum(np.abs(ref_rows[:, 1]), 1e-30)
+    assert rel.max() <= REL_TOL, f"{label}: mass rel error {rel.max()}"
+    if rows.shape[1] > 4:
+        a, b = rows[:, 4:], ref_rows[:, 4:]
+        err = np.abs(a - b) / np.maximum(np.abs(b), 1.0)
+        assert np.nanmax(err) <= REL_TOL, f"{label}: characterisation rel error {np.nanmax(err)}"
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("plain", [False, True])
+def test_synthetic_u16(ctx, frames512, case, plain):
+    kw = CASES[case]
+    print(f"[parity] synthetic u16 case={case} plain={plain}", flush=True)
+    rows, cand, cols, _ = locate(frames512, make_params(**kw), plain=plain, ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(frames512, kw)
+    _check(rows, cand, ref_rows, ref_cand, f"u16/{case}/plain={plain}")
+
+
+@pytest.mark.parametrize("case", ["c1", "defaults", "char", "even_d", "wide_smooth"])
+def test_float32_input(ctx, frames512, case):
+    kw = CASES[case]
+    f32 = frames512[:1].astype(np.float32) * np.float32(0.75)
+    print(f"[parity] float32 case={case}", flush=True)
+    rows, cand, _, _ = locate(f32, make_params(**kw), ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(f32, kw)
+    _check(rows, cand, ref_rows, ref_cand, f"f32/{case}")
+
+
+@pytest.mark.parametrize("case", ["c1", "defaults", "char", "small_sep"])
+def test_golden_grey_lion_u8(ctx, grey_lion, case):
+    kw = CASES[case]
+    print(f"[parity] grey_lion u8 case={case}", flush=True)
+    rows, cand, _, _ = locate(grey_lion[None], make_params(**kw), ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(grey_lion[None], kw)
+    _check(rows, cand, ref_rows, ref_cand, f"grey_lion/{case}")
+
+
+@pytest.mark.parametrize("shape", [(100, 130), (65, 257), (300, 64), (17, 23), (5, 5)])
+def test_odd_shapes(ctx, frames512, shape):
+    h, w = shape
+    crop = np.ascontiguousarray(frames512[:2, 37:37 + h, 11:11 + w])
+    for case in ("c1", "defaults", "char"):
+        kw = CASES[case]
+        print(f"[parity] odd shape {shape} case={case}", flush=True)
+        rows, cand, _, _ = locate(crop, make_params(**kw), ctx=ctx)
+        ref_rows, ref_cand = _oracle_movie(crop, kw)
+        _check(rows, cand, ref_rows, ref_cand, f"shape{shape}/{case}")
+
+
+def test_deterministic_and_plain_equals_batched(ctx, frames512):
+    p = make_params(**CASES["c2"])
+    a = locate(frames512, p, ctx=ctx)
+    b = locate(frames512, p, ctx=ctx)
+    c = locate(frames512, p, plain=True, ctx=ctx)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_empty_movie(ctx):
+    rows, cand, cols, _ = locate(np.zeros((0, 64, 64), np.uint16), make_params(9), ctx=ctx)
+    assert rows.shape == (0, 4) and len(cand) == 0
+    assert [c[0] for c in cols] == ["frame", "mass", "y", "x"]
+
+
+def test_blank_frames(ctx):
+    rows, cand, _, _ = locate(np.full((2, 128, 128), 7, np.uint16), make_params(9), ctx=ctx)
+    assert len(rows) == 0

@@ -1,0 +1,941 @@
+// gt_api.hip -- host side of libgpu_tracking.so: parameter handling, the
+// persistent device context, batch planning and the C ABI of
+// include/gpu_tracking.h.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gpu_tracking.h"
+#include "gt_internal.h"
+
+using namespace gt;
+
+namespace {
+
+thread_local std::string t_err;
+
+gt_status fail(gt_status s, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+gt_status fail(gt_status s, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? GT_E_OOM : GT_E_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+#define GT_TRY(expr)                  \
+    do {                              \
+        gt_status s_ = (expr);        \
+        if (s_ != GT_OK) return s_;   \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        if (n == 0) n = 256;
+        if (p && n <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        n = (n + 255) & ~(size_t)255;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+// ---- exact division check -------------------------------------------------
+
+__attribute__((target("fma"))) bool corrected_div_exact_fma(float d) {
+    const float rd = 1.0f / d;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t bits = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        const float q = x * rd;
+        const float r = __builtin_fmaf(-q, d, x);
+        const float q2 = __builtin_fmaf(r, rd, q);
+        if (q2 != x / d) return false;
+    }
+    return true;
+}
+
+bool corrected_div_exact_soft(float d) {
+    const float rd = 1.0f / d;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t bits = 0x3f800000u | m;
+        float x;
+        std::memcpy(&x, &bits, 4);
+        const float q = x * rd;
+        const float r = std::fma(-q, d, x);
+        const float q2 = std::fma(r, rd, q);
+        if (q2 != x / d) return false;
+    }
+    return true;
+}
+
+// Division x / d replaced by q = x*rd; r = fma(-q, d, x); q' = fma(r, rd, q)
+// with rd = RN(1/d) is correctly rounded for every x exactly when it is for
+// every significand in [1, 2) (the three steps commute with scaling by powers
+// of two while no result leaves the normal range).  Checked exhaustively,
+// cached per divisor.
+bool corrected_div_exact(float d) {
+    static std::mutex mu;
+    static std::map<uint32_t, bool> cache;
+    uint32_t key;
+    std::memcpy(&key, &d, 4);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    if (!(d > 0.f) || !std::isfinite(d) || !std::isnormal(d)) return false;
+    const bool ok = __builtin_cpu_supports("fma") ? corrected_div_exact_fma(d)
+                                                  : corrected_div_exact_soft(d);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = ok;
+    return ok;
+}
+
+// ---- parameters -----------------------------------------------------------
+
+gt_status check_params(const gt_params* p, uint32_t h, uint32_t w) {
+    if (!p) return fail(GT_E_INVALID, "params is NULL");
+    // lib.rs:73-74: not_implemented!(maxsize, threshold, invert, percentile, topn, preprocess)
+    if (p->has_maxsize) return fail(GT_E_NOT_IMPLEMENTED, "maxsize is not implemented");
+    if (p->has_threshold) return fail(GT_E_NOT_IMPLEMENTED, "threshold is not implemented");
+    if (p->has_invert) return fail(GT_E_NOT_IMPLEMENTED, "invert is not implemented");
+    if (p->has_topn) return fail(GT_E_NOT_IMPLEMENTED, "topn is not implemented");
+    if (p->has_preprocess) return fail(GT_E_NOT_IMPLEMENTED, "preprocess is not implemented");
+    if (p->has_search_range)
+        return fail(GT_E_NOT_IMPLEMENTED, "search_range (linking) is not implemented in this build");
+    if (p->has_sig_radius || p->has_bg_radius)
+        return fail(GT_E_NOT_IMPLEMENTED,
+                    "sig_radius/bg_radius (raw signal extraction) is not implemented in this build");
+    if (p->diameter < 1) return fail(GT_E_INVALID, "diameter must be >= 1");
+    if (p->smoothing_size < 1) return fail(GT_E_INVALID, "smoothing_size must be >= 1");
+    if (p->smoothing_size > 33) return fail(GT_E_INVALID, "smoothing_size must be <= 33");
+    if (!(p->noise_size > 0.f) || !std::isfinite(p->noise_size))
+        return fail(GT_E_INVALID, "noise_size must be positive and finite");
+    if (!std::isfinite(p->minmass)) return fail(GT_E_INVALID, "minmass must be finite");
+    if (p->has_percentile && !(p->percentile >= 0.f && p->percentile <= 100.f))
+        return fail(GT_E_INVALID, "percentile must be in [0, 100]");
+    if ((p->characterize || p->cpu_processed) && p->diameter > 31)
+        return fail(GT_E_INVALID, "characterize/cpu_processed need diameter <= 31");
+    if (p->max_iterations > 4096) return fail(GT_E_INVALID, "max_iterations must be <= 4096");
+    if (p->separation > 4096) return fail(GT_E_INVALID, "separation must be <= 4096");
+    if (h || w) {
+        const uint32_t R = p->smoothing_size / 2;
+        if (h < 1 || w < 1) return fail(GT_E_INVALID, "frame shape must be positive");
+        if (h < R || w < R)
+            return fail(GT_E_INVALID,
+                        "frame %ux%u is smaller than smoothing_size/2 = %u (single mirror "
+                        "reflection of the boxcar would leave the frame)",
+                        h, w, R);
+        if ((uint64_t)h * w >= (1ull << 31)) return fail(GT_E_INVALID, "frame too large");
+    }
+    return GT_OK;
+}
+
+int32_t ncols_of(const gt_params* p) {
+    return 4 + (p->cpu_processed ? 3 : 0) + (p->characterize ? 4 : 0);
+}
+
+// Derived geometry and host tables (gpu_setup.rs:45-60, kernels.rs:134-205).
+gt_status make_geom(const gt_params* p, bool integer_input, Geom& g) {
+    std::memset(&g, 0, sizeof(g));
+    g.k = (int32_t)p->smoothing_size;
+    g.R = g.k / 2;
+    g.band = (int32_t)(p->diameter / 2);
+    g.rint = ((int32_t)p->diameter - 1) / 2;
+    {
+        volatile float t = 2.0f * (float)p->separation;
+        t = t / std::sqrt(2.0f);
+        g.kd = (int32_t)(uint32_t)t;
+        volatile float hk = (float)g.kd / 2.0f;
+        hk = hk - 0.5f;
+        g.a = (int32_t)hk;
+    }
+    g.max_iterations = p->max_iterations;
+    g.minmass = p->minmass;
+    g.sep = (float)p->separation;
+    g.has_pct = p->has_percentile ? 1 : 0;
+    g.percentile = p->percentile;
+    g.characterize = p->characterize ? 1 : 0;
+    g.cpu_processed = p->cpu_processed ? 1 : 0;
+    g.disk_in_window = (g.rint <= g.a) && (g.rint <= g.kd - 1 - g.a);
+    // Gaussian weights (preprocess_rows.wgsl:44): argument in binary32,
+    // exponential evaluated in binary64 on the host and rounded once.
+    float gn = 0.f, wmin = 1.f;
+    for (int i = -g.R; i <= g.R; ++i) {
+        const float x = (float)i;
+        volatile float arg = -(x * x);
+        arg = arg / (2.0f * p->noise_size);
+        const float wi = (float)std::exp((double)arg);
+        g.w[i + g.R] = wi;
+        gn = gn + wi;
+        if (wi > 0.f) wmin = std::min(wmin, wi);
+    }
+    g.gn = gn;
+    g.rgn = 1.0f / gn;
+    g.kk = (float)(g.k * g.k);
+    g.rkk = 1.0f / g.kk;
+    g.fastdiv_gn = integer_input && wmin >= std::ldexp(1.0f, -40) && corrected_div_exact(gn);
+    g.fastdiv_kk = integer_input && corrected_div_exact(g.kk);
+    // dedup neighbourhood in tiles: start pixels of two rows closer than
+    // `separation` differ by less than sep + 2*(max_iterations + rint) per axis
+    const double D = std::ceil((double)p->separation) + 2.0 * ((double)p->max_iterations + g.rint) + 1.0;
+    g.dedup_rty = (int32_t)(D / TILE_H) + 1;
+    g.dedup_rtx = (int32_t)(D / TILE_W) + 1;
+    g.ncols = ncols_of(p);
+    // characterise tables (centers.wgsl:67-69 sin/cos of 2*atan2(j, i))
+    int t = 0;
+    for (int i = -g.rint; i <= g.rint; ++i)
+        for (int j = -g.rint; j <= g.rint; ++j) {
+            if (i * i + j * j > g.rint * g.rint) continue;
+            if (t >= 1024) return fail(GT_E_INVALID, "diameter too large for characterisation");
+            const double th = std::atan2((double)j, (double)i);
+            g.sin2_gpu[t] = (float)std::sin(2.0 * th);
+            g.cos2_gpu[t] = (float)std::cos(2.0 * th);
+            ++t;
+        }
+    g.ntap_gpu = t;
+    // cpu_processed tables over circle_inds(diameter/2) (kernels.rs:134-205)
+    t = 0;
+    for (int i = -g.band; i <= g.band; ++i)
+        for (int j = -g.band; j <= g.band; ++j) {
+            if (i * i + j * j > g.band * g.band) continue;
+            if (t >= 1024) return fail(GT_E_INVALID, "diameter too large for characterisation");
+            const double th = std::atan2((double)j, (double)i);
+            g.r2_cpu[t] = (float)(i * i + j * j);
+            g.sin_cpu[t] = (float)std::sin(2.0 * th);
+            g.cos_cpu[t] = (float)std::cos(2.0 * th);
+            ++t;
+        }
+    g.ntap_cpu = t;
+    return GT_OK;
+}
+
+const char* const kColFrame = "frame";
+
+}  // namespace
+
+// ---- context --------------------------------------------------------------
+
+struct gt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    gt_timing last{};
+    uint32_t batch_frames = 0;   // 0 = automatic
+    size_t workspace_limit = (size_t)6 << 30;
+    int seg_rows = 64;
+    // workspaces
+    DevBuf geom, frames_tab, bases, in_stage, P, tmp, list, slots, rec, keep;
+    DevBuf tile3, perframe, rows, rows_cand, counters;
+};
+
+struct gt_result {
+    uint64_t rows = 0;
+    uint32_t ncols = 0;
+    std::vector<float> data;
+    std::vector<int64_t> cand;
+    const float* ddata = nullptr;
+    std::vector<std::string> names, dtypes;
+};
+
+namespace {
+
+struct FrameIn {
+    const void* ptr;
+    uint32_t h, w;
+    int64_t fi;
+};
+
+size_t elem_size(int in_type) { return in_type == IN_U16 ? 2 : (in_type == IN_U8 ? 1 : 4); }
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct BatchPlan {
+    size_t b0, b1;
+};
+
+struct FrameSizes {
+    uint64_t px;
+    uint32_t k1_tx, k1_ty, k1_items;
+    uint32_t tx, ty, tiles;
+};
+
+FrameSizes sizes_of(uint32_t h, uint32_t w, int seg) {
+    FrameSizes s;
+    s.px = (uint64_t)h * w;
+    s.k1_tx = (w + K1_STRIP - 1) / K1_STRIP;
+    s.k1_ty = (h + seg - 1) / seg;
+    s.k1_items = s.k1_tx * s.k1_ty;
+    s.tx = (w + TILE_W - 1) / TILE_W;
+    s.ty = (h + TILE_H - 1) / TILE_H;
+    s.tiles = s.tx * s.ty;
+    return s;
+}
+
+int rec_planes(const Geom& g) {
+    if (g.characterize) return REC_NPLANES;
+    if (g.cpu_processed) return REC_SIGNAL + 1;
+    return REC_X + 1;
+}
+
+struct Need {
+    uint64_t px = 0, tiles = 0, k1 = 0, in_bytes = 0;
+    size_t nframes = 0;
+};
+
+size_t workspace_bytes(const Need& n, const Geom& g, bool generic) {
+    size_t b = 0;
+    b += al256(n.px * 4);                                   // P
+    if (generic) b += al256(n.px * 8);                      // tmp
+    if (g.has_pct) b += al256(n.px * 4);                    // list
+    b += al256(n.tiles * TILE_PX * 2);                      // slots
+    b += al256(n.tiles * TILE_PX * 4 * (uint64_t)rec_planes(g));  // rec
+    b += al256(n.tiles * TILE_PX);                          // keep
+    b += al256(n.in_bytes);                                 // input staging
+    return b;
+}
+
+gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
+                     const gt_params* p, uint32_t flags, gt_result** out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (!out) return fail(GT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    GT_TRY(check_params(p, 0, 0));
+    for (const auto& f : frames) {
+        GT_TRY(check_params(p, f.h, f.w));
+        if (!f.ptr) return fail(GT_E_INVALID, "frame pointer is NULL");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const bool on_device = flags & GT_FRAMES_ON_DEVICE;
+    const bool plain = flags & GT_PLAIN;
+    const bool result_on_device = flags & GT_RESULT_ON_DEVICE;
+    const bool integer_input = in_type != IN_F32;
+
+    Geom g;
+    GT_TRY(make_geom(p, integer_input, g));
+    const bool generic = g.R > 7;
+    const int seg = ctx->seg_rows;
+    const size_t es = elem_size(in_type);
+
+    gt_result* res = new gt_result();
+    res->ncols = (uint32_t)g.ncols;
+    {
+        const char* names[16];
+        const char* dtypes[16];
+        uint32_t nc = 0;
+        gt_column_names(p, names, dtypes, 16, &nc);
+        for (uint32_t i = 0; i < nc; ++i) {
+            res->names.push_back(names[i]);
+            res->dtypes.push_back(dtypes[i]);
+        }
+    }
+    ctx->last = gt_timing{};
+    ctx->last.frames = frames.size();
+    if (frames.empty()) {
+        *out = res;
+        return GT_OK;
+    }
+
+    // ---- batch plan -------------------------------------------------------
+    const size_t max_batch = plain ? 1 : (ctx->batch_frames ? ctx->batch_frames : 4096);
+    std::vector<BatchPlan> plan;
+    Need maxneed;
+    {
+        size_t b0 = 0;
+        while (b0 < frames.size()) {
+            Need n;
+            size_t b1 = b0;
+            while (b1 < frames.size() && (b1 - b0) < max_batch) {
+                const FrameSizes fs = sizes_of(frames[b1].h, frames[b1].w, seg);
+                Need m = n;
+                m.px += fs.px;
+                m.tiles += fs.tiles;
+                m.k1 += fs.k1_items;
+                m.in_bytes += on_device ? 0 : al256(fs.px * es);
+                m.nframes += 1;
+                if (b1 > b0 && workspace_bytes(m, g, generic) > ctx->workspace_limit) break;
+                n = m;
+                ++b1;
+            }
+            plan.push_back({b0, b1});
+            maxneed.px = std::max(maxneed.px, n.px);
+            maxneed.tiles = std::max(maxneed.tiles, n.tiles);
+            maxneed.k1 = std::max(maxneed.k1, n.k1);
+            maxneed.in_bytes = std::max(maxneed.in_bytes, n.in_bytes);
+            maxneed.nframes = std::max(maxneed.nframes, n.nframes);
+            b0 = b1;
+        }
+    }
+    uint64_t total_px = 0;
+    for (const auto& f : frames) total_px += (uint64_t)f.h * f.w;
+    ctx->last.pixels = total_px;
+
+    // ---- workspaces -------------------------------------------------------
+    const uint64_t slot_cap = maxneed.tiles * TILE_PX;
+    const size_t nfb = maxneed.nframes;
+    HIP_TRY(ctx->geom.ensure(sizeof(Geom)));
+    HIP_TRY(ctx->P.ensure(maxneed.px * 4));
+    if (generic) HIP_TRY(ctx->tmp.ensure(maxneed.px * 8));
+    if (g.has_pct) HIP_TRY(ctx->list.ensure(maxneed.px * 4));
+    HIP_TRY(ctx->slots.ensure(slot_cap * 2));
+    HIP_TRY(ctx->rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
+    HIP_TRY(ctx->keep.ensure(slot_cap));
+    if (!on_device) HIP_TRY(ctx->in_stage.ensure(maxneed.in_bytes));
+    HIP_TRY(ctx->tile3.ensure(al256(maxneed.tiles * 4) * 3));
+    // per-frame: hist (4096 u32), list_count, selstate(4), thr(double), frame_rows, done, row_base(u64)
+    const size_t pf_hist = al256(nfb * HIST_BINS * 4);
+    const size_t pf_small = al256(nfb * 4);
+    const size_t pf_sel = al256(nfb * 16);
+    const size_t pf_8 = al256(nfb * 8);
+    HIP_TRY(ctx->perframe.ensure(pf_hist + 3 * pf_small + pf_sel + 2 * pf_8));
+    HIP_TRY(ctx->counters.ensure(256));
+
+    // rows capacity: one row per 16 pixels, grown on overflow
+    uint64_t rows_cap = std::max<uint64_t>(4096, total_px / 16);
+    if (ctx->rows.bytes >= rows_cap * 4 * g.ncols && ctx->rows_cand.bytes >= rows_cap * 8) {
+        rows_cap = std::min<uint64_t>(ctx->rows.bytes / (4 * (uint64_t)g.ncols), ctx->rows_cand.bytes / 8);
+    }
+
+    // ---- frame tables for every batch (uploaded once) ---------------------
+    std::vector<Frame> ftab(frames.size());
+    std::vector<uint32_t> bases_all;  // per batch: k1 bases [nb+1] then tile bases [nb+1]
+    std::vector<size_t> bases_off(plan.size());
+    bool vec = true;
+    for (size_t bi = 0; bi < plan.size(); ++bi) {
+        const auto& bp = plan[bi];
+        uint64_t pxo = 0, inoff = 0;
+        uint32_t k1b = 0, tb = 0;
+        std::vector<uint32_t> k1v, tbv;
+        for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
+            const FrameSizes fs = sizes_of(frames[fi].h, frames[fi].w, seg);
+            Frame& F = ftab[fi];
+            std::memset(&F, 0, sizeof(F));
+            if (on_device) {
+                F.in = frames[fi].ptr;
+            } else {
+                F.in = (const char*)ctx->in_stage.p + inoff;
+                inoff += al256(fs.px * es);
+            }
+            F.P = ctx->P.as<float>() + pxo;
+            F.tmp = generic ? ctx->tmp.as<float>() + 2 * pxo : nullptr;
+            F.list_base = pxo;
+            F.H = frames[fi].h;
+            F.W = frames[fi].w;
+            F.k1_tx = fs.k1_tx;
+            F.k1_ty = fs.k1_ty;
+            F.tx = fs.tx;
+            F.ty = fs.ty;
+            F.tile_base = tb;
+            F.k1_base = k1b;
+            F.slot_base = (uint64_t)tb * TILE_PX;
+            F.frame_index = frames[fi].fi;
+            k1v.push_back(k1b);
+            tbv.push_back(tb);
+            k1b += fs.k1_items;
+            tb += fs.tiles;
+            // P is 16-byte aligned when every earlier frame of the batch has
+            // a pixel count that is a multiple of 4
+            if (F.W % 4 != 0 || (pxo % 4) != 0) vec = false;
+            const size_t align = es == 1 ? 4 : (es == 2 ? 8 : 16);
+            if (((uintptr_t)F.in) % align != 0) vec = false;
+            pxo += fs.px;
+        }
+        k1v.push_back(k1b);
+        tbv.push_back(tb);
+        bases_off[bi] = bases_all.size();
+        bases_all.insert(bases_all.end(), k1v.begin(), k1v.end());
+        bases_all.insert(bases_all.end(), tbv.begin(), tbv.end());
+    }
+    HIP_TRY(ctx->frames_tab.ensure(ftab.size() * sizeof(Frame)));
+    HIP_TRY(ctx->bases.ensure(bases_all.size() * 4));
+    HIP_TRY(hipMemcpy(ctx->frames_tab.p, ftab.data(), ftab.size() * sizeof(Frame), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->bases.p, bases_all.data(), bases_all.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->geom.p, &g, sizeof(Geom), hipMemcpyHostToDevice));
+    const Geom* d_geom = ctx->geom.as<const Geom>();
+
+    uint64_t launches = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        HIP_TRY(ctx->rows.ensure(rows_cap * 4 * (uint64_t)g.ncols));
+        HIP_TRY(ctx->rows_cand.ensure(rows_cap * 8));
+        uint64_t* counters = ctx->counters.as<uint64_t>();  // [0] cand_total, [1] row_carry
+        res->data.clear();
+        res->cand.clear();
+        launches = 0;
+
+        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(launch_fill_u64(counters, 2, 0, ctx->stream));
+        ++launches;
+        uint64_t host_rows_done = 0;
+        for (size_t bi = 0; bi < plan.size(); ++bi) {
+            const auto& bp = plan[bi];
+            const uint32_t nb = (uint32_t)(bp.b1 - bp.b0);
+            Batch b;
+            std::memset(&b, 0, sizeof(b));
+            b.frames = ctx->frames_tab.as<const Frame>() + bp.b0;
+            b.k1_bases = ctx->bases.as<const uint32_t>() + bases_off[bi];
+            b.tile_bases = b.k1_bases + nb + 1;
+            b.nframes = nb;
+            b.k1_items = bases_all[bases_off[bi] + nb];
+            b.tiles = bases_all[bases_off[bi] + nb + 1 + nb];
+            b.in_type = in_type;
+            b.vec = vec ? 1 : 0;
+            char* t3 = ctx->tile3.as<char>();
+            const size_t t3s = al256(maxneed.tiles * 4);
+            b.tile_count = (uint32_t*)t3;
+            b.tile_keep = (uint32_t*)(t3 + t3s);
+            b.tile_row_off = (uint32_t*)(t3 + 2 * t3s);
+            char* pf = ctx->perframe.as<char>();
+            b.hist1 = g.has_pct ? (uint32_t*)pf : nullptr;
+            pf += pf_hist;
+            b.list = g.has_pct ? ctx->list.as<uint32_t>() : nullptr;
+            b.list_count = (uint32_t*)pf; pf += pf_small;
+            b.frame_rows = (uint32_t*)pf; pf += pf_small;
+            b.done = (uint32_t*)pf; pf += pf_small;
+            b.selstate = (int32_t*)pf; pf += pf_sel;
+            b.thr = (double*)pf; pf += pf_8;
+            b.frame_row_base = (uint64_t*)pf;
+            b.slots = ctx->slots.as<uint16_t>();
+            b.rec = ctx->rec.as<float>();
+            b.keep = ctx->keep.as<uint8_t>();
+            b.slot_cap = slot_cap;
+            b.cand_total = counters;
+            b.row_carry = counters + 1;
+            b.rows = ctx->rows.as<float>();
+            b.rows_cand = ctx->rows_cand.as<int64_t>();
+            b.rows_cap = rows_cap;
+
+            if (!on_device) {
+                size_t off = 0;
+                for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
+                    const size_t nbytes = (size_t)frames[fi].h * frames[fi].w * es;
+                    HIP_TRY(hipMemcpyAsync((char*)ctx->in_stage.p + off, frames[fi].ptr, nbytes,
+                                           hipMemcpyHostToDevice, ctx->stream));
+                    off += al256(nbytes);
+                }
+            }
+            HIP_TRY(launch_zero_batch(b, ctx->stream));
+            ++launches;
+            if (generic) {
+                uint32_t maxpx = 0;
+                for (size_t fi = bp.b0; fi < bp.b1; ++fi)
+                    maxpx = std::max<uint32_t>(maxpx, frames[fi].h * frames[fi].w);
+                HIP_TRY(launch_bandpass_generic(b, d_geom, maxpx, g.has_pct, ctx->stream));
+                launches += 2;
+            } else {
+                HIP_TRY(launch_bandpass(b, g, d_geom, seg, ctx->stream));
+                ++launches;
+            }
+            if (g.has_pct) {
+                HIP_TRY(launch_select1(b, g, ctx->stream));
+                ++launches;
+            }
+            HIP_TRY(launch_detect(b, g, d_geom, ctx->stream));
+            ++launches;
+            if (g.has_pct) {
+                HIP_TRY(launch_select2(b, g, ctx->stream));
+                ++launches;
+            }
+            HIP_TRY(launch_refine(b, g, d_geom, ctx->stream));
+            HIP_TRY(launch_dedup(b, g, d_geom, ctx->stream));
+            HIP_TRY(launch_frame_scan(b, ctx->stream));
+            HIP_TRY(launch_emit(b, g, d_geom, ctx->stream));
+            launches += 4;
+
+            if (plain && !result_on_device) {
+                // per-frame readback, as the reference's get_work does
+                uint64_t carry = 0;
+                HIP_TRY(hipMemcpyAsync(&carry, counters + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                const uint64_t upto = std::min(carry, rows_cap);
+                if (upto > host_rows_done) {
+                    const uint64_t n = upto - host_rows_done;
+                    res->data.resize(upto * g.ncols);
+                    res->cand.resize(upto);
+                    HIP_TRY(hipMemcpyAsync(res->data.data() + host_rows_done * g.ncols,
+                                           ctx->rows.as<float>() + host_rows_done * g.ncols,
+                                           n * g.ncols * 4, hipMemcpyDeviceToHost, ctx->stream));
+                    HIP_TRY(hipMemcpyAsync(res->cand.data() + host_rows_done,
+                                           ctx->rows_cand.as<int64_t>() + host_rows_done, n * 8,
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                    HIP_TRY(hipStreamSynchronize(ctx->stream));
+                    host_rows_done = upto;
+                }
+            }
+        }
+        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        uint64_t cnt[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last.device_ms = ms;
+        ctx->last.candidates = cnt[0];
+        const uint64_t total_rows = cnt[1];
+        if (total_rows > rows_cap) {
+            rows_cap = total_rows + 1024;
+            continue;  // rerun with enough room (results are deterministic)
+        }
+        res->rows = total_rows;
+        if (result_on_device) {
+            res->ddata = ctx->rows.as<float>();
+        } else if (!plain) {
+            res->data.resize(total_rows * g.ncols);
+            res->cand.resize(total_rows);
+            if (total_rows) {
+                HIP_TRY(hipMemcpy(res->data.data(), ctx->rows.p, total_rows * g.ncols * 4,
+                                  hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(res->cand.data(), ctx->rows_cand.p, total_rows * 8,
+                                  hipMemcpyDeviceToHost));
+            }
+        }
+        break;
+    }
+    ctx->last.rows = res->rows;
+    ctx->last.launches = launches;
+    ctx->last.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    *out = res;
+    return GT_OK;
+}
+
+}  // namespace
+
+// ---- C ABI ----------------------------------------------------------------
+
+extern "C" {
+
+const char* gt_last_error(void) { return t_err.c_str(); }
+
+const char* gt_version(void) { return "gpu_tracking-mi355x 0.1.0 (gfx950, abi 1)"; }
+
+int gt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+gt_status gt_params_default(gt_params* p, uint32_t diameter) {
+    if (!p) return fail(GT_E_INVALID, "params is NULL");
+    std::memset(p, 0, sizeof(*p));
+    // lib.rs:77-91
+    p->diameter = diameter;
+    p->minmass = 0.f;
+    p->maxsize = INFINITY;
+    p->separation = diameter + 1;
+    p->noise_size = 1.f;
+    p->smoothing_size = diameter;
+    p->threshold = 1.f / 255.f;
+    p->invert = 0;
+    p->percentile = 64.f;
+    p->topn = UINT32_MAX;
+    p->preprocess = 1;
+    p->max_iterations = 10;
+    p->characterize = 0;
+    p->filter_close = 1;
+    p->cpu_processed = 0;
+    return GT_OK;
+}
+
+gt_status gt_params_rust_default(gt_params* p) {
+    if (!p) return fail(GT_E_INVALID, "params is NULL");
+    std::memset(p, 0, sizeof(*p));
+    // execute_gpu.rs:40-65
+    p->diameter = 9;
+    p->minmass = 0.f;
+    p->maxsize = 0.f;
+    p->separation = 11;
+    p->noise_size = 1.f;
+    p->smoothing_size = 9;
+    p->threshold = 0.f;
+    p->invert = 0;
+    p->percentile = 0.f;
+    p->topn = 0;
+    p->preprocess = 1;
+    p->max_iterations = 10;
+    p->characterize = 0;
+    p->filter_close = 1;
+    p->cpu_processed = 1;
+    return GT_OK;
+}
+
+gt_status gt_params_validate(const gt_params* p, uint32_t h, uint32_t w) {
+    return check_params(p, h, w);
+}
+
+gt_status gt_column_names(const gt_params* p, const char** names, const char** dtypes,
+                          uint32_t cap, uint32_t* ncols) {
+    if (!p || !ncols) return fail(GT_E_INVALID, "NULL argument");
+    static const char* const kInt = "int";
+    static const char* const kFloat = "float";
+    const char* n[16];
+    const char* d[16];
+    uint32_t c = 0;
+    auto push = [&](const char* name, const char* dt) { n[c] = name; d[c] = dt; ++c; };
+    // execute_gpu.rs:197-226
+    push(kColFrame, kInt);
+    push("mass", kFloat);
+    push("y", kFloat);
+    push("x", kFloat);
+    if (p->has_search_range) push("particle", kInt);
+    if (p->cpu_processed) {
+        push("Rg", kFloat);
+        push("ecc", kFloat);
+        push("signal", kFloat);
+    }
+    if (p->has_sig_radius) push("raw_mass", kFloat);
+    if (p->has_bg_radius) {
+        push("raw_bg_median", kFloat);
+        push("raw_mass_corrected", kFloat);
+    }
+    if (p->characterize) {
+        push("Rg_gpu", kFloat);
+        push("raw_mass_gpu", kFloat);
+        push("signal_gpu", kFloat);
+        push("ecc_gpu", kFloat);
+    }
+    *ncols = c;
+    for (uint32_t i = 0; i < c && i < cap; ++i) {
+        if (names) names[i] = n[i];
+        if (dtypes) dtypes[i] = d[i];
+    }
+    return c > cap ? fail(GT_E_INVALID, "capacity %u < %u columns", cap, c) : GT_OK;
+}
+
+gt_status gt_context_create(int device, gt_context** out) {
+    if (!out) return fail(GT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(GT_E_NO_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(GT_E_INVALID, "device %d out of range (%d)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    gt_context* c = new gt_context();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(GT_E_HIP, "stream/event creation failed");
+    }
+    *out = c;
+    return GT_OK;
+}
+
+gt_status gt_context_destroy(gt_context* ctx) {
+    if (!ctx) return GT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->in_stage, &ctx->P,
+                      &ctx->tmp, &ctx->list, &ctx->slots, &ctx->rec, &ctx->keep, &ctx->tile3,
+                      &ctx->perframe, &ctx->rows, &ctx->rows_cand, &ctx->counters};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GT_OK;
+}
+
+gt_status gt_context_timing(const gt_context* ctx, gt_timing* out) {
+    if (!ctx || !out) return fail(GT_E_INVALID, "NULL argument");
+    *out = ctx->last;
+    return GT_OK;
+}
+
+gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    ctx->batch_frames = frames_per_launch;
+    return GT_OK;
+}
+
+static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n, uint32_t h,
+                                uint32_t w, int in_type, const gt_params* p, uint32_t flags,
+                                gt_result** out) {
+    if (!frames && n) return fail(GT_E_INVALID, "frames is NULL");
+    std::vector<FrameIn> v;
+    v.reserve(n);
+    const size_t fb = (size_t)h * w * elem_size(in_type);
+    for (uint64_t i = 0; i < n; ++i) v.push_back({(const char*)frames + i * fb, h, w, (int64_t)i});
+    if (n == 0) {
+        GT_TRY(check_params(p, h, w));
+    }
+    return run_locate(ctx, v, in_type, p, flags, out);
+}
+
+gt_status gt_locate_u16(gt_context* ctx, const uint16_t* frames, uint64_t n_frames, uint32_t h,
+                        uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return locate_uniform(ctx, frames, n_frames, h, w, IN_U16, p, flags, out);
+}
+
+gt_status gt_locate_u8(gt_context* ctx, const uint8_t* frames, uint64_t n_frames, uint32_t h,
+                       uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return locate_uniform(ctx, frames, n_frames, h, w, IN_U8, p, flags, out);
+}
+
+gt_status gt_locate_f32(gt_context* ctx, const float* frames, uint64_t n_frames, uint32_t h,
+                        uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return locate_uniform(ctx, frames, n_frames, h, w, IN_F32, p, flags, out);
+}
+
+gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t n_frames,
+                          int dtype, const gt_params* p, uint32_t flags, gt_result** out) {
+    if (!frames && n_frames) return fail(GT_E_INVALID, "frames is NULL");
+    int in_type;
+    if (dtype == 0) in_type = IN_U16;
+    else if (dtype == 1) in_type = IN_F32;
+    else if (dtype == 2) in_type = IN_U8;
+    else return fail(GT_E_INVALID, "dtype must be 0 (u16), 1 (f32) or 2 (u8)");
+    std::vector<FrameIn> v;
+    v.reserve(n_frames);
+    for (uint64_t i = 0; i < n_frames; ++i)
+        v.push_back({frames[i].data, frames[i].h, frames[i].w, frames[i].frame_index});
+    return run_locate(ctx, v, in_type, p, flags, out);
+}
+
+uint64_t gt_result_rows(const gt_result* r) { return r ? r->rows : 0; }
+uint32_t gt_result_ncols(const gt_result* r) { return r ? r->ncols : 0; }
+const float* gt_result_data(const gt_result* r) {
+    return (r && !r->ddata) ? r->data.data() : nullptr;
+}
+const int64_t* gt_result_candidates(const gt_result* r) {
+    return (r && !r->ddata) ? r->cand.data() : nullptr;
+}
+const float* gt_result_device_data(const gt_result* r) { return r ? r->ddata : nullptr; }
+
+gt_status gt_result_columns(const gt_result* r, const char** names, const char** dtypes,
+                            uint32_t cap, uint32_t* ncols) {
+    if (!r || !ncols) return fail(GT_E_INVALID, "NULL argument");
+    *ncols = (uint32_t)r->names.size();
+    for (uint32_t i = 0; i < *ncols && i < cap; ++i) {
+        if (names) names[i] = r->names[i].c_str();
+        if (dtypes) dtypes[i] = r->dtypes[i].c_str();
+    }
+    return *ncols > cap ? fail(GT_E_INVALID, "capacity too small") : GT_OK;
+}
+
+void gt_result_free(gt_result* r) { delete r; }
+
+gt_status gt_synth_u16(gt_context* ctx, uint16_t* d_frames, uint64_t n_frames, uint32_t h,
+                       uint32_t w, const float* spots, const uint64_t* spot_off, const float* bg,
+                       uint64_t seed) {
+    if (!ctx || !d_frames || !spot_off || !bg) return fail(GT_E_INVALID, "NULL argument");
+    if (h == 0 || w == 0) return fail(GT_E_INVALID, "empty frame");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t cell = 32;
+    const uint32_t cx = (w + cell - 1) / cell, cy = (h + cell - 1) / cell;
+    const uint32_t ncell = cx * cy;
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(64, (1ull << 28) / ((uint64_t)h * w)));
+    DevBuf d_spots, d_off, d_bg;
+    for (uint64_t f0 = 0; f0 < n_frames; f0 += chunk) {
+        const uint64_t nf = std::min<uint64_t>(chunk, n_frames - f0);
+        // counting sort of each frame's spots into cells
+        std::vector<uint32_t> off(nf * (ncell + 1), 0);
+        const uint64_t s_begin = spot_off[f0], s_end = spot_off[f0 + nf];
+        std::vector<float4> sorted(s_end - s_begin);
+        for (uint64_t f = 0; f < nf; ++f) {
+            uint32_t* co = off.data() + f * (ncell + 1);
+            std::vector<uint32_t> cnt(ncell, 0);
+            for (uint64_t s = spot_off[f0 + f]; s < spot_off[f0 + f + 1]; ++s) {
+                const float y = spots[4 * s], x = spots[4 * s + 1];
+                int cyi = (int)std::floor(y / cell), cxi = (int)std::floor(x / cell);
+                cyi = std::max(0, std::min((int)cy - 1, cyi));
+                cxi = std::max(0, std::min((int)cx - 1, cxi));
+                cnt[(uint32_t)cyi * cx + (uint32_t)cxi]++;
+            }
+            uint32_t run = (uint32_t)(spot_off[f0 + f] - s_begin);
+            for (uint32_t c = 0; c < ncell; ++c) {
+                co[c] = run;
+                run += cnt[c];
+            }
+            co[ncell] = run;
+            std::vector<uint32_t> fill(co, co + ncell);
+            for (uint64_t s = spot_off[f0 + f]; s < spot_off[f0 + f + 1]; ++s) {
+                const float y = spots[4 * s], x = spots[4 * s + 1];
+                int cyi = (int)std::floor(y / cell), cxi = (int)std::floor(x / cell);
+                cyi = std::max(0, std::min((int)cy - 1, cyi));
+                cxi = std::max(0, std::min((int)cx - 1, cxi));
+                const uint32_t c = (uint32_t)cyi * cx + (uint32_t)cxi;
+                sorted[fill[c]++] = make_float4(y, x, spots[4 * s + 2], spots[4 * s + 3]);
+            }
+        }
+        HIP_TRY(d_spots.ensure(std::max<size_t>(16, sorted.size() * sizeof(float4))));
+        HIP_TRY(d_off.ensure(off.size() * 4));
+        HIP_TRY(d_bg.ensure(nf * 4));
+        if (!sorted.empty())
+            HIP_TRY(hipMemcpy(d_spots.p, sorted.data(), sorted.size() * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_off.p, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_bg.p, bg + f0, nf * 4, hipMemcpyHostToDevice));
+        HIP_TRY(launch_synth(d_frames + f0 * (uint64_t)h * w, nf, h, w, d_spots.as<const float4>(),
+                             d_off.as<const uint32_t>(), cx, cy, cell, d_bg.as<const float>(),
+                             seed, f0, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    d_spots.release();
+    d_off.release();
+    d_bg.release();
+    return GT_OK;
+}
+
+gt_status gt_device_alloc(gt_context* ctx, uint64_t bytes, void** out) {
+    if (!ctx || !out) return fail(GT_E_INVALID, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMalloc(out, bytes ? bytes : 1));
+    return GT_OK;
+}
+
+gt_status gt_device_free(gt_context* ctx, void* p) {
+    if (!ctx) return fail(GT_E_INVALID, "NULL argument");
+    if (p) HIP_TRY(hipFree(p));
+    return GT_OK;
+}
+
+gt_status gt_memcpy_h2d(gt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx) return fail(GT_E_INVALID, "NULL argument");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GT_OK;
+}
+
+gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx) return fail(GT_E_INVALID, "NULL argument");
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+gt_status gt_synchronize(gt_context* ctx) {
+    if (!ctx) return fail(GT_E_INVALID, "NULL argument");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return GT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// gt_internal.h -- device-side descriptors and launch entry points shared by
+// the HIP kernels (gt_kernels.hip) and the host orchestration (gt_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gt {
+
+// Output tile of the detect stage (K2).  It is also the grouping unit of
+// candidates: each tile owns TILE_H*TILE_W candidate slots, refinement,
+// deduplication and row emission all work tile by tile.
+constexpr int TILE_H = 64;
+constexpr int TILE_W = 64;
+constexpr int TILE_PX = TILE_H * TILE_W;
+
+// Bandpass (K1) column strip: 64 lanes x 4 columns.
+constexpr int K1_STRIP = 256;
+
+// Percentile histogram: 4096 bins over the float32 bit pattern
+// (bin 0 = exact zero, bin b >= 1 = bits >> 19 == b - 1).
+constexpr int HIST_BINS = 4096;
+
+enum InType : int { IN_U16 = 0, IN_F32 = 1, IN_U8 = 2 };
+
+// Record planes stored per candidate slot.
+enum RecPlane : int {
+    REC_MASS = 0, REC_Y = 1, REC_X = 2,
+    REC_RG = 3, REC_ECC = 4, REC_SIGNAL = 5,                 // cpu_processed
+    REC_RG_GPU = 6, REC_RAW_GPU = 7, REC_SIG_GPU = 8, REC_ECC_GPU = 9,  // characterize
+    REC_NPLANES = 10
+};
+
+struct Frame {
+    const void* in;        // H*W input pixels (device)
+    float* P;              // H*W bandpassed frame
+    float* tmp;            // 2*H*W row-pass temporary (generic bandpass only)
+    uint64_t slot_base;    // first candidate slot of this frame
+    uint32_t H, W;
+    uint32_t k1_tx, k1_ty; // bandpass strips x segments
+    uint32_t tx, ty;       // detect tiles
+    uint32_t tile_base;    // first detect tile of this frame in the batch
+    uint32_t k1_base;      // first bandpass item of this frame
+    int64_t frame_index;
+    uint64_t list_base;    // percentile list offset (H*W capacity)
+};
+
+// Parameters derived once per call (gpu_setup.rs:45-60 plus host tables).
+struct Geom {
+    int32_t k;             // smoothing taps parameter
+    int32_t R;             // k / 2
+    int32_t band;          // diameter / 2
+    int32_t rint;          // (diameter - 1) / 2
+    int32_t kd;            // dilation size
+    int32_t a;             // window start offset
+    uint32_t max_iterations;
+    float minmass;
+    float sep;             // dedup radius
+    int32_t has_pct;
+    float percentile;
+    int32_t characterize;
+    int32_t cpu_processed;
+    int32_t disk_in_window;
+    int32_t fastdiv_gn;    // division by gauss_norm via corrected reciprocal
+    int32_t fastdiv_kk;    // division by k*k via corrected reciprocal
+    float w[33];           // Gaussian weights, taps -R..R (R <= 16)
+    float gn, rgn;         // gauss_norm and its rounded reciprocal
+    float kk, rkk;         // float(k*k) and its rounded reciprocal
+    int32_t dedup_rty, dedup_rtx;  // tile radius of the dedup neighbourhood
+    int32_t ncols;         // output columns
+    int32_t ntap_gpu;      // disk taps (characterize tables)
+    int32_t ntap_cpu;      // circle taps (cpu_processed tables)
+    float sin2_gpu[1024], cos2_gpu[1024];                // disk radius rint
+    float r2_cpu[1024], sin_cpu[1024], cos_cpu[1024];    // circle radius band
+};
+
+struct Batch {
+    const Frame* frames;   // device array [nframes]
+    const uint32_t* k1_bases;   // [nframes + 1]
+    const uint32_t* tile_bases; // [nframes + 1]
+    uint32_t nframes;
+    uint32_t k1_items;
+    uint32_t tiles;
+    int in_type;           // InType
+    int vec;               // W % 4 == 0 and aligned frame bases: vector loads
+    // per-tile arrays [tiles]
+    uint32_t* tile_count;
+    uint32_t* tile_keep;
+    uint32_t* tile_row_off;
+    // per-frame arrays [nframes]
+    uint32_t* hist1;       // [nframes * HIST_BINS]
+    uint32_t* list;        // percentile candidates (bit patterns)
+    uint32_t* list_count;
+    int32_t* selstate;     // [nframes * 4]: bin_lo, rank_lo, bin_hi, rank_hi
+    double* thr;           // [nframes]
+    uint32_t* frame_rows;  // [nframes]
+    uint32_t* done;        // [nframes] last-block counters
+    uint64_t* frame_row_base;  // [nframes]
+    // candidate slots and records
+    uint16_t* slots;       // [tiles * TILE_PX]
+    float* rec;            // [REC_NPLANES * slot_cap]
+    uint8_t* keep;         // [slot_cap]
+    uint64_t slot_cap;
+    // output
+    uint64_t* cand_total;  // device running candidate count (statistics)
+    uint64_t* row_carry;   // device running row count across batches
+    float* rows;
+    int64_t* rows_cand;
+    uint64_t rows_cap;
+};
+
+// Launchers (gt_kernels.hip).  All launch on `s`; return hipSuccess or the
+// first launch error.
+hipError_t launch_zero_batch(const Batch& b, hipStream_t s);
+hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom,
+                           int seg_rows, hipStream_t s);
+hipError_t launch_bandpass_generic(const Batch& b, const Geom* d_geom, uint32_t max_px,
+                                   int hist, hipStream_t s);
+hipError_t launch_select1(const Batch& b, const Geom& g, hipStream_t s);
+hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom,
+                         hipStream_t s);
+hipError_t launch_select2(const Batch& b, const Geom& g, hipStream_t s);
+hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom,
+                         hipStream_t s);
+hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom,
+                        hipStream_t s);
+hipError_t launch_frame_scan(const Batch& b, hipStream_t s);
+hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
+                       hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+
+// Synthetic frames (gt_synth.hip).
+hipError_t launch_synth(uint16_t* frames, uint64_t nframes, uint32_t h,
+                        uint32_t w, const float4* spots, const uint32_t* cell_off,
+                        uint32_t cells_x, uint32_t cells_y, uint32_t cell,
+                        const float* bg, uint64_t seed, uint64_t frame0,
+                        hipStream_t s);
+
+}  // namespace gt

@@ -1,0 +1,218 @@
+"""Python API of the MI355X locate path, mirroring the reference package
+``gpu_tracking`` (gpu-tracking/gpu_tracking/lib.py:6-106 and the PyO3 entry
+points of gpu-tracking/src/lib.rs:46-200).
+
+``batch(video, diameter, ...)`` and ``batch_file(path, diameter, ...)`` take the
+reference's keyword arguments with the same defaults and return a DataFrame
+with the reference's columns and dtypes (execute_gpu.rs:197-226).
+
+Differences, all supersets:
+  * ``percentile`` is implemented (the reference panics, lib.rs:73-74): when
+    given, a local maximum is kept only when its bandpassed value is strictly
+    above the exact ``percentile``-th percentile (linear interpolation) of all
+    bandpassed pixels of its frame;
+  * ``video`` may be float32 (the reference's type), uint16 or uint8, as a
+    numpy array or a torch tensor (device tensors are read in place);
+  * errors raise ``GtInvalid`` (a ValueError) or ``GtNotImplemented`` (a
+    NotImplementedError) instead of a Rust panic.
+Not implemented in this build (raise GtNotImplemented): ``search_range``
+(linking), ``sig_radius``/``bg_radius`` (raw signal extraction) and the
+parameters the reference itself rejects (maxsize, threshold, invert, topn,
+preprocess).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import (GT_FRAMES_ON_DEVICE, GT_PLAIN, GtError, GtInvalid, GtNotImplemented,
+                   GtParams, Result, check, default_context, load)
+
+__all__ = ["batch", "batch_rust", "batch_file", "batch_file_rust", "make_params",
+           "locate", "column_names", "GtError", "GtInvalid", "GtNotImplemented"]
+
+
+def make_params(diameter, minmass=None, maxsize=None, separation=None, noise_size=None,
+                smoothing_size=None, threshold=None, invert=None, percentile=None, topn=None,
+                preprocess=None, max_iterations=None, characterize=None, filter_close=None,
+                search_range=None, memory=None, cpu_processed=None, sig_radius=None,
+                bg_radius=None, gap_radius=None) -> GtParams:
+    """Build gt_params from batch_rust's keyword arguments (lib.rs:77-91)."""
+    if diameter is None or int(diameter) != diameter or diameter < 0:
+        raise GtInvalid(_lib.GT_E_INVALID, "diameter must be a non-negative integer")
+    p = GtParams()
+    check(load().gt_params_default(ctypes.byref(p), int(diameter)))
+    if minmass is not None:
+        p.minmass = float(minmass)
+    if maxsize is not None:
+        p.has_maxsize, p.maxsize = 1, float(maxsize)
+    if separation is not None:
+        p.separation = int(separation)
+    if noise_size is not None:
+        p.noise_size = float(noise_size)
+    if smoothing_size is not None:
+        p.smoothing_size = int(smoothing_size)
+    if threshold is not None:
+        p.has_threshold, p.threshold = 1, float(threshold)
+    if invert is not None:
+        p.has_invert, p.invert = 1, int(bool(invert))
+    if percentile is not None:
+        p.has_percentile, p.percentile = 1, float(percentile)
+    if topn is not None:
+        p.has_topn, p.topn = 1, int(topn)
+    if preprocess is not None:
+        p.has_preprocess, p.preprocess = 1, int(bool(preprocess))
+    if max_iterations is not None:
+        p.max_iterations = int(max_iterations)
+    if characterize is not None:
+        p.characterize = int(bool(characterize))
+    if filter_close is not None:
+        p.filter_close = int(bool(filter_close))
+    if search_range is not None:
+        p.has_search_range, p.search_range = 1, float(search_range)
+    if memory is not None:
+        p.has_memory, p.memory = 1, int(memory)
+    if cpu_processed is not None:
+        p.cpu_processed = int(bool(cpu_processed))
+    if sig_radius is not None:
+        p.has_sig_radius, p.sig_radius = 1, float(sig_radius)
+    if bg_radius is not None:
+        p.has_bg_radius, p.bg_radius = 1, float(bg_radius)
+        # lib.rs:91: gap_radius defaults to 0 when bg_radius is given
+        p.has_gap_radius, p.gap_radius = 1, float(gap_radius if gap_radius is not None else 0.0)
+    elif gap_radius is not None:
+        p.has_gap_radius, p.gap_radius = 1, float(gap_radius)
+    return p
+
+
+def column_names(p: GtParams) -> list[tuple[str, str]]:
+    names = (ctypes.c_char_p * 16)()
+    dts = (ctypes.c_char_p * 16)()
+    n = ctypes.c_uint32()
+    check(load().gt_column_names(ctypes.byref(p), names, dts, 16, ctypes.byref(n)))
+    return [(names[i].decode(), dts[i].decode()) for i in range(n.value)]
+
+
+def _frames_arg(video):
+    """(pointer, n, h, w, dtype, on_device, keepalive) for a 3-D movie."""
+    try:
+        import torch  # noqa: F401
+        is_torch = type(video).__module__.startswith("torch")
+    except ImportError:
+        is_torch = False
+    if is_torch:
+        import torch
+        if video.dim() == 2:
+            video = video.unsqueeze(0)
+        if video.dim() != 3:
+            raise GtInvalid(_lib.GT_E_INVALID, "video must be (frames, height, width)")
+        if not video.is_contiguous():
+            raise GtInvalid(_lib.GT_E_INVALID, "Array is not standard layout")
+        dt = {torch.uint16: np.uint16, torch.uint8: np.uint8, torch.float32: np.float32}.get(video.dtype)
+        if dt is None:
+            raise GtInvalid(_lib.GT_E_INVALID, f"unsupported dtype {video.dtype}")
+        on_dev = video.is_cuda
+        if not on_dev:
+            arr = video.numpy()
+            return arr.ctypes.data, arr.shape[0], arr.shape[1], arr.shape[2], np.dtype(dt), False, arr
+        return video.data_ptr(), video.shape[0], video.shape[1], video.shape[2], np.dtype(dt), True, video
+    arr = np.asarray(video)
+    if arr.ndim == 2:
+        arr = arr[None]
+    if arr.ndim != 3:
+        raise GtInvalid(_lib.GT_E_INVALID, "video must be (frames, height, width)")
+    if not arr.flags.c_contiguous:
+        # execute_gpu.rs:497-499 panics on non-standard layout
+        raise GtInvalid(_lib.GT_E_INVALID, "Array is not standard layout")
+    if arr.dtype not in _lib.DTYPE_CODES:
+        if arr.dtype.kind == "f":
+            raise GtInvalid(_lib.GT_E_INVALID, "float video must be float32")
+        raise GtInvalid(_lib.GT_E_INVALID, f"unsupported dtype {arr.dtype}")
+    return arr.ctypes.data, arr.shape[0], arr.shape[1], arr.shape[2], arr.dtype, False, arr
+
+
+def locate(video, params: GtParams, plain: bool = False, device: int = 0, ctx=None):
+    """Run the locate path; returns (rows float32 [N, C], candidate pixel index
+    int64 [N], columns, timing dict)."""
+    ptr, n, h, w, dt, on_dev, keep = _frames_arg(video)
+    ctx = ctx or default_context(device)
+    flags = (GT_FRAMES_ON_DEVICE if on_dev else 0) | (GT_PLAIN if plain else 0)
+    fn = {np.dtype(np.uint16): load().gt_locate_u16, np.dtype(np.uint8): load().gt_locate_u8,
+          np.dtype(np.float32): load().gt_locate_f32}[dt]
+    out = ctypes.c_void_p()
+    if on_dev:
+        import torch
+        torch.cuda.current_stream(video.device).synchronize()
+    check(fn(ctx.handle, ctypes.c_void_p(ptr), n, h, w, ctypes.byref(params), flags, ctypes.byref(out)))
+    del keep
+    res = Result(out)
+    return res.rows(), res.candidates(), res.columns(), ctx.timing()
+
+
+def batch_rust(video, diameter, minmass=None, maxsize=None, separation=None, noise_size=None,
+               smoothing_size=None, threshold=None, invert=None, percentile=None, topn=None,
+               preprocess=None, max_iterations=None, characterize=None, filter_close=None,
+               search_range=None, memory=None, cpu_processed=None, sig_radius=None,
+               bg_radius=None, gap_radius=None):
+    """lib.rs:46-121: returns (float32 array [N, C], [(name, dtype), ...])."""
+    p = make_params(diameter, minmass, maxsize, separation, noise_size, smoothing_size, threshold,
+                    invert, percentile, topn, preprocess, max_iterations, characterize,
+                    filter_close, search_range, memory, cpu_processed, sig_radius, bg_radius,
+                    gap_radius)
+    check(load().gt_params_validate(ctypes.byref(p), 0, 0))
+    rows, _cand, cols, _t = locate(video, p)
+    return rows, cols
+
+
+def batch(video, diameter, minmass=None, maxsize=None, separation=None, noise_size=None,
+          smoothing_size=None, threshold=None, invert=None, percentile=None, topn=None,
+          preprocess=None, max_iterations=None, characterize=None, filter_close=None,
+          search_range=None, memory=None, cpu_processed=None, sig_radius=None, bg_radius=None,
+          gap_radius=None):
+    """lib.py:6-54: locate every frame of ``video`` and return a DataFrame."""
+    import pandas as pd
+    arr, columns = batch_rust(video, diameter, minmass, maxsize, separation, noise_size,
+                              smoothing_size, threshold, invert, percentile, topn, preprocess,
+                              max_iterations, characterize, filter_close, search_range, memory,
+                              cpu_processed, sig_radius, bg_radius, gap_radius)
+    columns = {name: typ for name, typ in columns}
+    return pd.DataFrame(arr, columns=columns).astype(columns)
+
+
+def batch_file_rust(filename, diameter, channel=None, minmass=None, maxsize=None, separation=None,
+                    noise_size=None, smoothing_size=None, threshold=None, invert=None,
+                    percentile=None, topn=None, preprocess=None, max_iterations=None,
+                    characterize=None, filter_close=None, search_range=None, memory=None,
+                    cpu_processed=None, sig_radius=None, bg_radius=None, gap_radius=None):
+    """lib.rs:123-200 for TIFF files (execute_gpu.rs:510-542)."""
+    from .tiff import read_tiff
+    ext = os.path.splitext(filename)[1].lower().lstrip(".")
+    if ext in ("tif", "tiff"):
+        video = read_tiff(filename)
+    elif ext == "ets":
+        raise GtNotImplemented(_lib.GT_E_NOT_IMPLEMENTED, "ets files are not supported in this build")
+    else:
+        raise GtInvalid(_lib.GT_E_INVALID, f"File extension '{ext}' not supported")
+    return batch_rust(video, diameter, minmass, maxsize, separation, noise_size, smoothing_size,
+                      threshold, invert, percentile, topn, preprocess, max_iterations,
+                      characterize, filter_close, search_range, memory, cpu_processed, sig_radius,
+                      bg_radius, gap_radius)
+
+
+def batch_file(path, diameter, channel=None, minmass=None, maxsize=None, separation=None,
+               noise_size=None, smoothing_size=None, threshold=None, invert=None, percentile=None,
+               topn=None, preprocess=None, max_iterations=None, characterize=None,
+               filter_close=None, search_range=None, memory=None, cpu_processed=None,
+               sig_radius=None, bg_radius=None, gap_radius=None):
+    """lib.py:56-106."""
+    import pandas as pd
+    arr, columns = batch_file_rust(path, diameter, channel, minmass, maxsize, separation,
+                                   noise_size, smoothing_size, threshold, invert, percentile, topn,
+                                   preprocess, max_iterations, characterize, filter_close,
+                                   search_range, memory, cpu_processed, sig_radius, bg_radius,
+                                   gap_radius)
+    columns = {name: typ for name, typ in columns}
+    return pd.DataFrame(arr, columns=columns).astype(columns)
