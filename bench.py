@@ -1,0 +1,232 @@
+"""Benchmark of the locate path on one MI355X.
+
+Workload: a device-resident movie of 1000 frames of 2048x2048 uint16 (5000
+Gaussian spots per frame, sigma 2 px, lognormal peaks with median 1200,
+Brownian motion D = 0.5 px^2/frame, background 150 + Poisson noise), located
+with diameter 11, separation 11, smoothing_size 11, percentile 64, minmass 0.
+
+A step is one pass of the locate path over the whole movie.  `value` is the
+batched path's frames/s with frames and result rows resident in HBM; the plain
+per-frame path (launch + completion wait per frame, as the reference's
+execute_gpu loop does) is timed on the same movie beside it.  The roofline
+entry is for the dominant kernel, measured with HIP events on the stream the
+library launches it on; the CPU baseline is the oracle (the CPU restatement of
+the reference pipeline) on a bounded sample of the same frames.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(h, w, n_cand, n_rows, ncols, rint):
+    """Bytes of the three-kernel locate pipeline per frame, each counted once:
+    u16 frame read (2), float32 bandpass written (4) and read once (4) per
+    pixel; per candidate its index (4) and one (2r+1)^2 box read; 4 bytes per
+    output value."""
+    return 10.0 * h * w + n_cand * (4.0 + 4.0 * (2 * rint + 1) ** 2) + 4.0 * ncols * n_rows
+
+
+def cpu_baseline(frames_host: np.ndarray, kw: dict, threads: int):
+    """Oracle frames/s over the sample, frames spread over `threads` threads
+    (ctypes releases the GIL, so the threads run in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle
+    p = oracle.Params(**kw)
+    n = len(frames_host)
+    f32 = [frames_host[i].astype(np.float32) for i in range(n)]
+
+    def one(i):
+        return len(oracle.locate_frame(f32[i], p, frame_index=i)[0])
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        rows = sum(ex.map(one, range(n)))
+    dt = time.perf_counter() - t0
+    return n / dt, rows, dt
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=1000)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=0, help="frames per launch (0 = automatic)")
+    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": f"--gpus {args.gpus}: this benchmark runs on exactly one GPU; "
+                                   "frames are independent, so N GPUs would each run this same "
+                                   "path on 1/N of the movie"}))
+        return 2
+
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import make_params
+
+    kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    params = make_params(**kw)
+    ctx = _lib.Context(0)
+    if args.batch:
+        ctx.set_batch(args.batch)
+    L = _lib.load()
+    import ctypes
+
+    H = W = args.size
+    nf = args.frames
+    log(f"generating {nf} frames of {H}x{W} uint16 on the device")
+    t0 = time.perf_counter()
+    spec = synth.movie_2048(seed=2, n_frames=nf, size=H)
+    movie = synth.generate(ctx, spec)
+    ctx.synchronize()
+    log(f"movie ready in {time.perf_counter() - t0:.1f} s ({movie.nbytes / 1e9:.2f} GB)")
+
+    def run(flags):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), nf, H, W,
+                                   ctypes.byref(params), flags, ctypes.byref(out)))
+        r = _lib.Result(out)
+        t = ctx.timing()
+        del r
+        return t
+
+    batched_flags = _lib.GT_FRAMES_ON_DEVICE | _lib.GT_RESULT_ON_DEVICE
+    plain_flags = batched_flags | _lib.GT_PLAIN
+
+    def sync():
+        ctx.synchronize()
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+        except ImportError:
+            pass
+
+    for i in range(args.warmup):
+        t = run(batched_flags)
+        log(f"warmup {i}: device {t['device_ms']:.1f} ms, {t['candidates']} candidates, "
+            f"{t['rows']} rows")
+    sync()
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        stats.append(run(batched_flags))
+    sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    fps = nf * args.steps / elapsed
+    dev_ms = float(np.mean([s["device_ms"] for s in stats]))
+    n_cand = stats[-1]["candidates"]
+    n_rows = stats[-1]["rows"]
+    log(f"batched: {fps:.0f} frames/s ({ms_per_step:.1f} ms/step, device {dev_ms:.1f} ms), "
+        f"{n_cand / nf:.0f} candidates/frame, {n_rows / nf:.0f} rows/frame")
+
+    # profiled step: per-stage kernel durations on the library's stream
+    ctx.set_profiling(True)
+    run(batched_flags)
+    stages = ctx.stage_times()
+    ctx.set_profiling(False)
+    for k, (ms, n) in stages.items():
+        if n:
+            log(f"  stage {k:9s}: {ms:8.3f} ms total over {n} launches ({ms / n:.4f} ms/launch)")
+    dominant = max(stages, key=lambda k: stages[k][0])
+    dom_ms, dom_n = stages[dominant]
+    rint = (kw["diameter"] - 1) // 2
+    px = float(H) * W * nf
+    # algorithmic bytes of each kernel over the whole step
+    kernel_bytes = {
+        "bandpass": px * (2 + 4),                       # u16 in, float32 out
+        "detect": px * 4 + n_cand * 2,                  # float32 in, candidate slots out
+        "refine": n_cand * (2 + 4 * (2 * rint + 1) ** 2 + 12),
+        "dedup": n_cand * 16.0,
+        "emit": n_rows * 4.0 * 4 + n_rows * 8.0,
+    }
+    dom_bytes = kernel_bytes.get(dominant, 0.0)
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    per_launch_bytes = dom_bytes / max(dom_n, 1)
+    pipe_bytes = algorithmic_bytes(H, W, n_cand / nf, n_rows / nf, 4, rint) * nf
+    pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
+    log(f"dominant kernel {dominant}: {achieved:.0f} GB/s over {per_launch_bytes / 1e6:.1f} MB "
+        f"per launch; pipeline {pipe_gbs:.0f} GB/s of algorithmic bytes "
+        f"({100 * pipe_gbs / HBM_PEAK_GBS:.1f} % of 8 TB/s)")
+
+    plain = None
+    if not args.no_plain:
+        run(plain_flags)
+        sync()
+        t0 = time.perf_counter()
+        tp = run(plain_flags)
+        sync()
+        pe = time.perf_counter() - t0
+        plain = {"frames_per_s": nf / pe, "ms_per_frame": 1000.0 * pe / nf,
+                 "launches": tp["launches"]}
+        log(f"plain per-frame path: {nf / pe:.0f} frames/s ({1000 * pe / nf:.3f} ms/frame); "
+            f"batched / plain = {fps / (nf / pe):.2f}x")
+
+    cpu = None
+    if not args.no_cpu and args.cpu_sample > 0:
+        ns = min(args.cpu_sample, nf)
+        sample = movie.to_host(np.uint16, (ns, H, W))
+        cores = len(os.sched_getaffinity(0))
+        threads = max(1, min(cores, 16))
+        okw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64.0, minmass=0.0)
+        cfps, crows, cdt = cpu_baseline(sample, okw, threads)
+        log(f"cpu oracle: {cfps:.2f} frames/s over {ns} frames on {threads} threads ({cdt:.1f} s)")
+        cpu = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} frames of the same movie, oracle binary32 mode, "
+                         f"{threads} threads (one frame per thread)"}
+
+    line = {
+        "metric": "frames/s at 2048^2 uint16 (~5k spots) batched vs plain; HBM GB/s % of peak",
+        "value": fps,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded device generator), device-resident",
+        "config": {"workload": f"{nf} frames {H}x{W} uint16, 5000 spots/frame, sigma 2, "
+                               "Brownian D=0.5, bg 150 + Poisson",
+                   "params": kw, "path": "batched", "frames": nf},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": per_launch_bytes,
+                     "ms_per_launch": dom_ms / max(dom_n, 1)},
+        "pipeline": {"algorithmic_GBps": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
+                     "candidates_per_frame": n_cand / nf, "rows_per_frame": n_rows / nf,
+                     "device_ms_per_step": dev_ms,
+                     "stage_ms": {k: v[0] for k, v in stages.items() if v[1]}},
+        "plain": plain,
+        "batched_over_plain": (fps / plain["frames_per_s"]) if plain else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

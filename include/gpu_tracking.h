@@ -145,6 +145,24 @@ gt_status gt_context_timing(const gt_context* ctx, gt_timing* out);
 /* Frames per launch of the batched path (0 = automatic). */
 gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch);
 
+/* Per-stage kernel timing (HIP events recorded around every launch on the
+ * context's stream).  Stages, in pipeline order: */
+#define GT_STAGE_ZERO 0      /* clear per-frame counters */
+#define GT_STAGE_BANDPASS 1  /* preprocess_rows.wgsl + preprocess_cols.wgsl */
+#define GT_STAGE_SELECT1 2   /* percentile: histogram bin of the quantile */
+#define GT_STAGE_DETECT 3    /* max_rows.wgsl + walk_cols.wgsl is_max */
+#define GT_STAGE_SELECT2 4   /* percentile: exact order statistics */
+#define GT_STAGE_REFINE 5    /* walk + centers.wgsl at visited pixels */
+#define GT_STAGE_DEDUP 6     /* execute_gpu.rs:258-284 */
+#define GT_STAGE_SCAN 7      /* row offsets */
+#define GT_STAGE_EMIT 8      /* rows in column_names order */
+#define GT_NSTAGES 9
+/* enable != 0 turns event timing on for the following locate calls. */
+gt_status gt_context_set_profiling(gt_context* ctx, int enable);
+/* Sum of kernel durations (ms) and launch counts per stage since profiling
+ * was last enabled; arrays of GT_NSTAGES entries. */
+gt_status gt_context_stage_times(const gt_context* ctx, double* ms, uint64_t* launches);
+
 /* --- locate ------------------------------------------------------------- */
 
 /* execute_ndarray (execute_gpu.rs:496) / batch_rust (lib.rs:46) for a

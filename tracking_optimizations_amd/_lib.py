@@ -36,8 +36,10 @@ EXPORTS = [
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
-    "gt_memcpy_d2h", "gt_synchronize",
+    "gt_memcpy_d2h", "gt_synchronize", "gt_context_set_profiling", "gt_context_stage_times",
 ]
+
+STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
 
 
 class GtError(RuntimeError):
@@ -172,6 +174,8 @@ def load() -> ctypes.CDLL:
         L.gt_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_synchronize.argtypes = [vp]
+        L.gt_context_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.gt_context_stage_times.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64)]
         _lib = L
         return L
 
@@ -228,6 +232,16 @@ class Context:
 
     def synchronize(self) -> None:
         check(load().gt_synchronize(self.handle))
+
+    def set_profiling(self, enable: bool) -> None:
+        check(load().gt_context_set_profiling(self.handle, int(bool(enable))))
+
+    def stage_times(self) -> dict:
+        """{stage: (total ms, launches)} accumulated since profiling was enabled."""
+        ms = (ctypes.c_double * len(STAGES))()
+        n = (ctypes.c_uint64 * len(STAGES))()
+        check(load().gt_context_stage_times(self.handle, ms, n))
+        return {s: (ms[i], int(n[i])) for i, s in enumerate(STAGES)}
 
 
 _tls = threading.local()

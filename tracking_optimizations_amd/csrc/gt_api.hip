@@ -255,6 +255,12 @@ struct gt_context {
     uint32_t batch_frames = 0;   // 0 = automatic
     size_t workspace_limit = (size_t)6 << 30;
     int seg_rows = 64;
+    // per-stage event timing
+    int profiling = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+    double stage_ms[GT_NSTAGES] = {0};
+    uint64_t stage_launches[GT_NSTAGES] = {0};
     // workspaces
     DevBuf geom, frames_tab, bases, in_stage, P, tmp, list, slots, rec, keep;
     DevBuf tile3, perframe, rows, rows_cand, counters;
@@ -324,6 +330,47 @@ size_t workspace_bytes(const Need& n, const Geom& g, bool generic) {
     b += al256(n.tiles * TILE_PX);                          // keep
     b += al256(n.in_bytes);                                 // input staging
     return b;
+}
+
+hipEvent_t take_event(gt_context* ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Runs one stage launch; with profiling on, brackets it with events.
+template <class F>
+hipError_t stage(gt_context* ctx, int id, F&& launch) {
+    if (!ctx->profiling) return launch();
+    hipEvent_t a = take_event(ctx), b = take_event(ctx);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(a, ctx->stream);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(b, ctx->stream);
+    ctx->ev_used.push_back({id, {a, b}});
+    return e;
+}
+
+// After a stream synchronisation: fold the recorded event pairs into the
+// per-stage totals and recycle the events.
+void collect_stage_times(gt_context* ctx) {
+    for (auto& u : ctx->ev_used) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, u.second.first, u.second.second) == hipSuccess) {
+            ctx->stage_ms[u.first] += ms;
+            ctx->stage_launches[u.first] += 1;
+        }
+        ctx->ev_pool.push_back(u.second.first);
+        ctx->ev_pool.push_back(u.second.second);
+    }
+    ctx->ev_used.clear();
 }
 
 gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
@@ -545,35 +592,41 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                     off += al256(nbytes);
                 }
             }
-            HIP_TRY(launch_zero_batch(b, ctx->stream));
+            HIP_TRY(stage(ctx, GT_STAGE_ZERO, [&] { return launch_zero_batch(b, ctx->stream); }));
             ++launches;
             if (generic) {
                 uint32_t maxpx = 0;
                 for (size_t fi = bp.b0; fi < bp.b1; ++fi)
                     maxpx = std::max<uint32_t>(maxpx, frames[fi].h * frames[fi].w);
-                HIP_TRY(launch_bandpass_generic(b, d_geom, maxpx, g.has_pct, ctx->stream));
+                HIP_TRY(stage(ctx, GT_STAGE_BANDPASS, [&] {
+                    return launch_bandpass_generic(b, d_geom, maxpx, g.has_pct, ctx->stream);
+                }));
                 launches += 2;
             } else {
-                HIP_TRY(launch_bandpass(b, g, d_geom, seg, ctx->stream));
+                HIP_TRY(stage(ctx, GT_STAGE_BANDPASS,
+                              [&] { return launch_bandpass(b, g, d_geom, seg, ctx->stream); }));
                 ++launches;
             }
             if (g.has_pct) {
-                HIP_TRY(launch_select1(b, g, ctx->stream));
+                HIP_TRY(stage(ctx, GT_STAGE_SELECT1, [&] { return launch_select1(b, g, ctx->stream); }));
                 ++launches;
             }
-            HIP_TRY(launch_detect(b, g, d_geom, ctx->stream));
+            HIP_TRY(stage(ctx, GT_STAGE_DETECT, [&] { return launch_detect(b, g, d_geom, ctx->stream); }));
             ++launches;
             if (g.has_pct) {
-                HIP_TRY(launch_select2(b, g, ctx->stream));
+                HIP_TRY(stage(ctx, GT_STAGE_SELECT2, [&] { return launch_select2(b, g, ctx->stream); }));
                 ++launches;
             }
-            HIP_TRY(launch_refine(b, g, d_geom, ctx->stream));
-            HIP_TRY(launch_dedup(b, g, d_geom, ctx->stream));
-            HIP_TRY(launch_frame_scan(b, ctx->stream));
-            HIP_TRY(launch_emit(b, g, d_geom, ctx->stream));
+            HIP_TRY(stage(ctx, GT_STAGE_REFINE, [&] { return launch_refine(b, g, d_geom, ctx->stream); }));
+            HIP_TRY(stage(ctx, GT_STAGE_DEDUP, [&] { return launch_dedup(b, g, d_geom, ctx->stream); }));
+            HIP_TRY(stage(ctx, GT_STAGE_SCAN, [&] { return launch_frame_scan(b, ctx->stream); }));
+            HIP_TRY(stage(ctx, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, ctx->stream); }));
             launches += 4;
 
-            if (plain && !result_on_device) {
+            if (plain && result_on_device) {
+                // per-frame completion, as the reference's get_work waits per frame
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+            } else if (plain) {
                 // per-frame readback, as the reference's get_work does
                 uint64_t carry = 0;
                 HIP_TRY(hipMemcpyAsync(&carry, counters + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -601,6 +654,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last.device_ms = ms;
+        collect_stage_times(ctx);
         ctx->last.candidates = cnt[0];
         const uint64_t total_rows = cnt[1];
         if (total_rows > rows_cap) {
@@ -760,6 +814,8 @@ gt_status gt_context_destroy(gt_context* ctx) {
                       &ctx->tmp, &ctx->list, &ctx->slots, &ctx->rec, &ctx->keep, &ctx->tile3,
                       &ctx->perframe, &ctx->rows, &ctx->rows_cand, &ctx->counters};
     for (DevBuf* b : bufs) b->release();
+    collect_stage_times(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipStreamDestroy(ctx->stream);
@@ -770,6 +826,25 @@ gt_status gt_context_destroy(gt_context* ctx) {
 gt_status gt_context_timing(const gt_context* ctx, gt_timing* out) {
     if (!ctx || !out) return fail(GT_E_INVALID, "NULL argument");
     *out = ctx->last;
+    return GT_OK;
+}
+
+gt_status gt_context_set_profiling(gt_context* ctx, int enable) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    ctx->profiling = enable ? 1 : 0;
+    for (int i = 0; i < GT_NSTAGES; ++i) {
+        ctx->stage_ms[i] = 0.0;
+        ctx->stage_launches[i] = 0;
+    }
+    return GT_OK;
+}
+
+gt_status gt_context_stage_times(const gt_context* ctx, double* ms, uint64_t* launches) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    for (int i = 0; i < GT_NSTAGES; ++i) {
+        if (ms) ms[i] = ctx->stage_ms[i];
+        if (launches) launches[i] = ctx->stage_launches[i];
+    }
     return GT_OK;
 }
 
