@@ -324,7 +324,7 @@ size_t workspace_bytes(const Need& n, const Geom& g, bool generic) {
     size_t b = 0;
     b += al256(n.px * 4);                                   // P
     if (generic) b += al256(n.px * 8);                      // tmp
-    if (g.has_pct) b += al256(n.px * 4);                    // list
+    if (g.has_pct) b += al256(n.tiles * TILE_PX * 4);       // list
     b += al256(n.tiles * TILE_PX * 2);                      // slots
     b += al256(n.tiles * TILE_PX * 4 * (uint64_t)rec_planes(g));  // rec
     b += al256(n.tiles * TILE_PX);                          // keep
@@ -392,9 +392,22 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
 
     Geom g;
     GT_TRY(make_geom(p, integer_input, g));
-    const bool generic = g.R > 7;
     const int seg = ctx->seg_rows;
     const size_t es = elem_size(in_type);
+    // vector loads need W % 4 == 0 and frame bases aligned to 4 pixels
+    bool vec = true;
+    for (const auto& f : frames) {
+        if (f.w % 4 != 0) vec = false;
+        if (on_device && ((uintptr_t)f.ptr) % (4 * es) != 0) vec = false;
+    }
+    bool generic;
+    {
+        Batch probe;
+        std::memset(&probe, 0, sizeof(probe));
+        probe.in_type = in_type;
+        probe.vec = vec ? 1 : 0;
+        generic = !bandpass_fast_path(probe, g);
+    }
 
     gt_result* res = new gt_result();
     res->ncols = (uint32_t)g.ncols;
@@ -455,13 +468,13 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     HIP_TRY(ctx->geom.ensure(sizeof(Geom)));
     HIP_TRY(ctx->P.ensure(maxneed.px * 4));
     if (generic) HIP_TRY(ctx->tmp.ensure(maxneed.px * 8));
-    if (g.has_pct) HIP_TRY(ctx->list.ensure(maxneed.px * 4));
+    if (g.has_pct) HIP_TRY(ctx->list.ensure(slot_cap * 4));
     HIP_TRY(ctx->slots.ensure(slot_cap * 2));
     HIP_TRY(ctx->rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
     HIP_TRY(ctx->keep.ensure(slot_cap));
     if (!on_device) HIP_TRY(ctx->in_stage.ensure(maxneed.in_bytes));
-    HIP_TRY(ctx->tile3.ensure(al256(maxneed.tiles * 4) * 3));
-    // per-frame: hist (4096 u32), list_count, selstate(4), thr(double), frame_rows, done, row_base(u64)
+    HIP_TRY(ctx->tile3.ensure(al256(maxneed.tiles * 4) * 4));
+    // per-frame: hist (4096 u32), frame_rows, frame_cands, spare, selstate(4), thr(double), row_base(u64)
     const size_t pf_hist = al256(nfb * HIST_BINS * 4);
     const size_t pf_small = al256(nfb * 4);
     const size_t pf_sel = al256(nfb * 16);
@@ -479,7 +492,6 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     std::vector<Frame> ftab(frames.size());
     std::vector<uint32_t> bases_all;  // per batch: k1 bases [nb+1] then tile bases [nb+1]
     std::vector<size_t> bases_off(plan.size());
-    bool vec = true;
     for (size_t bi = 0; bi < plan.size(); ++bi) {
         const auto& bp = plan[bi];
         uint64_t pxo = 0, inoff = 0;
@@ -497,7 +509,6 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             }
             F.P = ctx->P.as<float>() + pxo;
             F.tmp = generic ? ctx->tmp.as<float>() + 2 * pxo : nullptr;
-            F.list_base = pxo;
             F.H = frames[fi].h;
             F.W = frames[fi].w;
             F.k1_tx = fs.k1_tx;
@@ -512,11 +523,6 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             tbv.push_back(tb);
             k1b += fs.k1_items;
             tb += fs.tiles;
-            // P is 16-byte aligned when every earlier frame of the batch has
-            // a pixel count that is a multiple of 4
-            if (F.W % 4 != 0 || (pxo % 4) != 0) vec = false;
-            const size_t align = es == 1 ? 4 : (es == 2 ? 8 : 16);
-            if (((uintptr_t)F.in) % align != 0) vec = false;
             pxo += fs.px;
         }
         k1v.push_back(k1b);
@@ -563,13 +569,14 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             b.tile_count = (uint32_t*)t3;
             b.tile_keep = (uint32_t*)(t3 + t3s);
             b.tile_row_off = (uint32_t*)(t3 + 2 * t3s);
+            b.tile_lcount = (uint32_t*)(t3 + 3 * t3s);
             char* pf = ctx->perframe.as<char>();
             b.hist1 = g.has_pct ? (uint32_t*)pf : nullptr;
             pf += pf_hist;
             b.list = g.has_pct ? ctx->list.as<uint32_t>() : nullptr;
-            b.list_count = (uint32_t*)pf; pf += pf_small;
             b.frame_rows = (uint32_t*)pf; pf += pf_small;
-            b.done = (uint32_t*)pf; pf += pf_small;
+            b.frame_cands = (uint32_t*)pf; pf += pf_small;
+            pf += pf_small;
             b.selstate = (int32_t*)pf; pf += pf_sel;
             b.thr = (double*)pf; pf += pf_8;
             b.frame_row_base = (uint64_t*)pf;
@@ -619,9 +626,10 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             }
             HIP_TRY(stage(ctx, GT_STAGE_REFINE, [&] { return launch_refine(b, g, d_geom, ctx->stream); }));
             HIP_TRY(stage(ctx, GT_STAGE_DEDUP, [&] { return launch_dedup(b, g, d_geom, ctx->stream); }));
+            HIP_TRY(stage(ctx, GT_STAGE_SCAN, [&] { return launch_tile_scan(b, ctx->stream); }));
             HIP_TRY(stage(ctx, GT_STAGE_SCAN, [&] { return launch_frame_scan(b, ctx->stream); }));
             HIP_TRY(stage(ctx, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, ctx->stream); }));
-            launches += 4;
+            launches += 5;
 
             if (plain && result_on_device) {
                 // per-frame completion, as the reference's get_work waits per frame

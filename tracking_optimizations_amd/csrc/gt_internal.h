@@ -42,7 +42,6 @@ struct Frame {
     uint32_t tile_base;    // first detect tile of this frame in the batch
     uint32_t k1_base;      // first bandpass item of this frame
     int64_t frame_index;
-    uint64_t list_base;    // percentile list offset (H*W capacity)
 };
 
 // Parameters derived once per call (gpu_setup.rs:45-60 plus host tables).
@@ -87,14 +86,14 @@ struct Batch {
     uint32_t* tile_count;
     uint32_t* tile_keep;
     uint32_t* tile_row_off;
+    uint32_t* tile_lcount; // percentile list entries of the tile
     // per-frame arrays [nframes]
     uint32_t* hist1;       // [nframes * HIST_BINS]
-    uint32_t* list;        // percentile candidates (bit patterns)
-    uint32_t* list_count;
+    uint32_t* list;        // percentile candidates (bit patterns), TILE_PX per tile
     int32_t* selstate;     // [nframes * 4]: bin_lo, rank_lo, bin_hi, rank_hi
     double* thr;           // [nframes]
     uint32_t* frame_rows;  // [nframes]
-    uint32_t* done;        // [nframes] last-block counters
+    uint32_t* frame_cands; // [nframes] candidates entering refinement
     uint64_t* frame_row_base;  // [nframes]
     // candidate slots and records
     uint16_t* slots;       // [tiles * TILE_PX]
@@ -112,6 +111,9 @@ struct Batch {
 // Launchers (gt_kernels.hip).  All launch on `s`; return hipSuccess or the
 // first launch error.
 hipError_t launch_zero_batch(const Batch& b, hipStream_t s);
+// The streaming bandpass handles integer input with 1 <= R <= 7, W % 4 == 0
+// and aligned frame bases (b.vec); everything else takes the generic kernels.
+bool bandpass_fast_path(const Batch& b, const Geom& g);
 hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom,
                            int seg_rows, hipStream_t s);
 hipError_t launch_bandpass_generic(const Batch& b, const Geom* d_geom, uint32_t max_px,
@@ -124,6 +126,7 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom,
                         hipStream_t s);
+hipError_t launch_tile_scan(const Batch& b, hipStream_t s);
 hipError_t launch_frame_scan(const Batch& b, hipStream_t s);
 hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
                        hipStream_t s);

@@ -112,18 +112,14 @@ __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
 // ---------------------------------------------------------------------------
 __global__ void k_zero_batch(Batch b, int with_hist) {
     const uint64_t nh = with_hist ? (uint64_t)b.nframes * HIST_BINS : 0;
-    const uint64_t total = nh + 2ull * b.nframes;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        if (i < nh) b.hist1[i] = 0;
-        else if (i < nh + b.nframes) b.list_count[i - nh] = 0;
-        else b.done[i - nh - b.nframes] = 0;
-    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        b.hist1[i] = 0;
 }
 
 hipError_t launch_zero_batch(const Batch& b, hipStream_t s) {
     const int with_hist = b.hist1 != nullptr;
-    const uint64_t total = (with_hist ? (uint64_t)b.nframes * HIST_BINS : 0) + 2ull * b.nframes;
+    const uint64_t total = with_hist ? (uint64_t)b.nframes * HIST_BINS : 0;
     uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 4096);
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_zero_batch, dim3(blocks), dim3(256), 0, s, b, with_hist);
@@ -147,30 +143,90 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
 // K1 bandpass, streaming column strips.
 //
 // A work item is a 256-column strip of `seg` output rows of one frame.  The
-// single wave walks down the strip: every input row (reflected at the frame
-// edge) is staged once in LDS, each lane computes the row pass of its four
-// columns (Gaussian with out-of-bounds taps skipped, boxcar over mirrored
-// columns) and keeps the last 2R+1 row results in a register ring; once the
-// ring is full the column pass produces one output row.  For integer input
-// with k <= 15 the boxcar sums are exact integers in binary32, so the
-// vertical boxcar is a running sum; otherwise both boxcar passes add the taps
-// in the shader's order.
+// single wave walks down the strip: every input row (reflected at the top and
+// bottom edges) is staged once in LDS, each lane computes the row pass of its
+// four columns and keeps the last 2R+1 row results in a register ring; once
+// the ring is full the column pass produces one output row.  Input rows are
+// fetched 2R+1 rows ahead with 8-byte (u16) / 4-byte (u8) vector loads of
+// 4-column chunks.  With W % 4 == 0 no chunk straddles a frame edge, so a
+// chunk outside the frame is the mirror image of an in-frame chunk: it is
+// loaded from there and reversed when staged.  Two staged copies of the row
+// implement the two boundary rules: out-of-frame columns are zero in the
+// Gaussian copy (taps skipped, Q3) and mirrored in the boxcar copy (Q4).
+//
+// Integer input with k <= 15 keeps every boxcar sum an exact integer below
+// 2^24 in binary32, so the horizontal boxcar slides and the vertical boxcar is
+// a running sum.  FAST selects the corrected-reciprocal division by the two
+// per-call divisors (host-verified to equal IEEE division).
 // ---------------------------------------------------------------------------
+// Pointers fetched from the frame table are generic; marking them global
+// lets the compiler emit global (not flat) memory instructions, whose waits
+// do not also drain the LDS counter.  Only the device pass knows the
+// address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL __attribute__((address_space(1)))
+#else
+#define GLOBAL
+#endif
+
 struct K1Params {
     float w[15];
     float gn, rgn, kk, rkk;
-    int fast_gn, fast_kk;
     int seg;
     int hist;
-    int vec;
 };
 
-template <int R, typename T>
+template <typename T> struct Chunk4;
+template <> struct Chunk4<uint16_t> { uint2 v; };
+template <> struct Chunk4<uint8_t> { uint32_t v; };
+
+template <typename T>
+__device__ __forceinline__ Chunk4<T> ld_chunk(const GLOBAL T* row, int src) {
+    Chunk4<T> c;
+    if constexpr (sizeof(T) == 2) c.v = *(const GLOBAL uint2*)(row + src);
+    else c.v = *(const GLOBAL uint32_t*)(row + src);
+    return c;
+}
+
+template <typename T>
+__device__ __forceinline__ float4 unpack_chunk(const Chunk4<T>& c, bool rev) {
+    float4 r;
+    if constexpr (sizeof(T) == 2) {
+        r = make_float4((float)(c.v.x & 0xffffu), (float)(c.v.x >> 16),
+                        (float)(c.v.y & 0xffffu), (float)(c.v.y >> 16));
+    } else {
+        r = make_float4((float)(c.v & 0xffu), (float)((c.v >> 8) & 0xffu),
+                        (float)((c.v >> 16) & 0xffu), (float)(c.v >> 24));
+    }
+    if (rev) r = make_float4(r.w, r.z, r.y, r.x);
+    return r;
+}
+
+// Where the 4-column chunk starting at column c (c % 4 == 0) comes from:
+// its own columns, or (outside the frame) the mirrored in-frame chunk, read
+// in reverse.  Chunks that no output can reach are clamped into the frame.
+__device__ __forceinline__ void chunk_src(int c, int W, int& src, bool& rev, bool& oob) {
+    if (c < 0) { src = -c - 4; rev = true; oob = true; }
+    else if (c >= W) { src = 2 * W - c - 4; rev = true; oob = true; }
+    else { src = c; rev = false; oob = false; }
+    src = max(0, min(src, W - 4));
+}
+
+__device__ __forceinline__ float cdiv_t(float x, float d, float rd, bool fast) {
+    if (fast) {
+        const float q = x * rd;
+        const float r = __builtin_fmaf(-q, d, x);
+        return __builtin_fmaf(r, rd, q);
+    }
+    return x / d;
+}
+
+template <int R, typename T, bool FAST>
 __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     constexpr int NT = 2 * R + 1;
-    constexpr bool EXACT = !IsFloat<T>::v;
     constexpr int LB = K1_STRIP + 16;
-    __shared__ __attribute__((aligned(16))) float rowbuf[2][LB];
+    __shared__ __attribute__((aligned(16))) float bufg[2][LB];
+    __shared__ __attribute__((aligned(16))) float bufb[2][LB];
     __shared__ uint32_t s_hist[HIST_BINS];
 
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
@@ -183,11 +239,18 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     const int y0 = sy * kp.seg;
     const int y1 = min(y0 + kp.seg, H);
     const int t = threadIdx.x;
-    const T* __restrict__ in = (const T*)fr.in;
-    float* __restrict__ P = fr.P;
-    const bool interior = (x0 - R >= 0) && (x0 + K1_STRIP + R <= W);
+    const GLOBAL T* in = (const GLOBAL T*)fr.in;
+    GLOBAL float* P = (GLOBAL float*)fr.P;
+    // strip touches a frame edge (halo or main chunks outside the frame)
+    const bool border = (x0 - 8 < 0) || (x0 + K1_STRIP + 8 > W);
     const int cb = x0 + 4 * t;
-    const bool vec_main = kp.vec && (x0 + K1_STRIP <= W);
+    // main chunk of this lane and (lanes 0..3) one halo chunk
+    int msrc, hsrc;
+    bool mrev, moob, hrev, hoob;
+    chunk_src(cb, W, msrc, mrev, moob);
+    const int hc = (t < 2) ? (x0 - 8 + 4 * t) : (x0 + K1_STRIP + 4 * (t - 2));
+    const int hoff = (t < 2) ? (4 * t) : (8 + K1_STRIP + 4 * (t - 2));
+    chunk_src(hc, W, hsrc, hrev, hoob);
 
     if (kp.hist) {
         for (int i = t; i < HIST_BINS; i += 64) s_hist[i] = 0;
@@ -206,117 +269,90 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     uint32_t zc = 0;
 
     const int vr0 = y0 - R, vr1 = y1 + R;
-
-    // prefetch registers for one input row: 4 main columns + (lanes 0..3)
-    // 4 halo columns.
-    float pm[4], ph[4];
-    auto load_row = [&](int vr) {
+    auto row_ptr = [&](int vr) {
         const int ar = vr < 0 ? -vr - 1 : (vr >= H ? 2 * H - vr - 1 : vr);
-        const T* row = in + (size_t)ar * (size_t)W;
-        if (vec_main) {
-            if constexpr (sizeof(T) == 2) {
-                const uint2 u = *(const uint2*)(row + cb);
-                pm[0] = (float)(u.x & 0xffffu); pm[1] = (float)(u.x >> 16);
-                pm[2] = (float)(u.y & 0xffffu); pm[3] = (float)(u.y >> 16);
-            } else if constexpr (sizeof(T) == 1) {
-                const uint32_t u = *(const uint32_t*)(row + cb);
-                pm[0] = (float)(u & 0xffu); pm[1] = (float)((u >> 8) & 0xffu);
-                pm[2] = (float)((u >> 16) & 0xffu); pm[3] = (float)(u >> 24);
-            } else {
-                const float4 u = *(const float4*)(row + cb);
-                pm[0] = u.x; pm[1] = u.y; pm[2] = u.z; pm[3] = u.w;
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                int c = cb + q;
-                if (c >= W) c = 2 * W - c - 1;
-                c = max(0, min(c, W - 1));
-                pm[q] = ld_px(row, (size_t)c);
-            }
-        }
-        if (t < 4) {
-            const int hb = (t < 2) ? (x0 - 8 + 4 * t) : (x0 + K1_STRIP + 4 * (t - 2));
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                int c = hb + q;
-                if (c < 0) c = -c - 1;
-                else if (c >= W) c = 2 * W - c - 1;
-                c = max(0, min(c, W - 1));
-                ph[q] = ld_px(row, (size_t)c);
-            }
-        }
+        return in + (size_t)ar * (size_t)W;
     };
 
-    load_row(vr0);
+    // prefetch ring: row vr lives in slot (vr - vr0) % NT until it is staged
+    Chunk4<T> pm[NT], ph[NT];
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+        if (vr0 + s < vr1) {
+            const GLOBAL T* row = row_ptr(vr0 + s);
+            pm[s] = ld_chunk<T>(row, msrc);
+            if (t < 4) ph[s] = ld_chunk<T>(row, hsrc);
+        }
+    }
+
     int parity = 0;
     for (int base = vr0; base < vr1; base += NT) {
 #pragma unroll
         for (int s = 0; s < NT; ++s) {
             const int vr = base + s;
             if (vr < vr1) {  // wave-uniform
-            float* buf = rowbuf[parity];
+            float* bg_ = bufg[parity];
+            float* bb_ = bufb[parity];
             parity ^= 1;
-            *(float4*)(buf + 8 + 4 * t) = make_float4(pm[0], pm[1], pm[2], pm[3]);
-            if (t < 4) {
-                const int hi = (t < 2) ? (4 * t) : (8 + K1_STRIP + 4 * (t - 2));
-                *(float4*)(buf + hi) = make_float4(ph[0], ph[1], ph[2], ph[3]);
+            {
+                const float4 m = unpack_chunk<T>(pm[s], mrev);
+                *(float4*)(bg_ + 8 + 4 * t) = moob ? make_float4(0.f, 0.f, 0.f, 0.f) : m;
+                *(float4*)(bb_ + 8 + 4 * t) = m;
+                if (t < 4) {
+                    const float4 h = unpack_chunk<T>(ph[s], hrev);
+                    *(float4*)(bg_ + hoff) = hoob ? make_float4(0.f, 0.f, 0.f, 0.f) : h;
+                    *(float4*)(bb_ + hoff) = h;
+                }
             }
-            if (vr + 1 < vr1) load_row(vr + 1);
+            if (vr + NT < vr1) {
+                const GLOBAL T* row = row_ptr(vr + NT);
+                pm[s] = ld_chunk<T>(row, msrc);
+                if (t < 4) ph[s] = ld_chunk<T>(row, hsrc);
+            }
             __syncthreads();
 
             float v[20];
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                const float4 x = *(const float4*)(buf + 4 * t + 4 * m);
+                const float4 x = *(const float4*)(bg_ + 4 * t + 4 * m);
                 v[4 * m + 0] = x.x; v[4 * m + 1] = x.y; v[4 * m + 2] = x.z; v[4 * m + 3] = x.w;
             }
-            const bool row_in = (vr >= 0) && (vr < H);
             float tg[4], tb[4];
-            if (interior) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float gs = 0.f;
+            for (int q = 0; q < 4; ++q) {
+                float gs = 0.f;
 #pragma unroll
-                    for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], v[8 + q + i], gs);
-                    tg[q] = gs;
+                for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], v[8 + q + i], gs);
+                tg[q] = gs;
+            }
+            float vb[20];
+            if (border) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) {
+                    const float4 x = *(const float4*)(bb_ + 4 * t + 4 * m);
+                    vb[4 * m + 0] = x.x; vb[4 * m + 1] = x.y; vb[4 * m + 2] = x.z; vb[4 * m + 3] = x.w;
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float gs = 0.f;
-#pragma unroll
-                    for (int i = -R; i <= R; ++i) {
-                        const int c = cb + q + i;
-                        if (c >= 0 && c < W) gs = __builtin_fmaf(w[i + R], v[8 + q + i], gs);
-                    }
-                    tg[q] = gs;
-                }
+                for (int m = 0; m < 20; ++m) vb[m] = v[m];
             }
-            if constexpr (EXACT) {
+            {
                 float bx = 0.f;
 #pragma unroll
-                for (int i = -R; i <= R; ++i) bx = bx + v[8 + i];
+                for (int i = -R; i <= R; ++i) bx = bx + vb[8 + i];
                 tb[0] = bx;
 #pragma unroll
                 for (int q = 1; q < 4; ++q) {
-                    bx = (bx + v[8 + q + R]) - v[8 + q - 1 - R];
-                    tb[q] = bx;
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float bx = 0.f;
-#pragma unroll
-                    for (int i = -R; i <= R; ++i) bx = bx + v[8 + q + i];
+                    bx = (bx + vb[8 + q + R]) - vb[8 + q - 1 - R];
                     tb[q] = bx;
                 }
             }
+            const bool row_in = (vr >= 0) && (vr < H);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float g = row_in ? cdiv(tg[q], kp.gn, kp.rgn, kp.fast_gn) : 0.f;
-                if constexpr (EXACT) vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
-                rg[s][q] = g;
+                const float gq = row_in ? cdiv_t(tg[q], kp.gn, kp.rgn, FAST) : 0.f;
+                vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
+                rg[s][q] = gq;
                 rb[s][q] = tb[q];
             }
 
@@ -329,34 +365,20 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
 #pragma unroll
                     for (int j = 0; j < NT; ++j)
                         gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
-                    float box;
-                    if constexpr (EXACT) {
-                        box = vbox[q];
-                    } else {
-                        box = 0.f;
-#pragma unroll
-                        for (int j = 0; j < NT; ++j) box = box + rb[(s + 1 + j) % NT][q];
-                    }
-                    const float res = cdiv(gsum, kp.gn, kp.rgn, kp.fast_gn) -
-                                      cdiv(box, kp.kk, kp.rkk, kp.fast_kk);
+                    const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
+                                      cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
                     out[q] = (res < 0.f) ? 0.f : res;
                 }
-                float* prow = P + (size_t)o * (size_t)W;
-                if (vec_main) {
-                    *(float4*)(prow + cb) = make_float4(out[0], out[1], out[2], out[3]);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (cb + q < W) prow[cb + q] = out[q];
+                if (cb + 3 < W) {
+                    *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cb) =
+                        make_float4(out[0], out[1], out[2], out[3]);
                 }
-                if (kp.hist) {
+                if (kp.hist && cb + 3 < W) {
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        if (cb + q < W) {
-                            const uint32_t bits = __float_as_uint(out[q]);
-                            if (bits == 0) zc++;
-                            else atomicAdd(&s_hist[pct_key(bits)], 1u);
-                        }
+                        const uint32_t bits = __float_as_uint(out[q]);
+                        zc += bits == 0 ? 1u : 0u;
+                        if (bits != 0) atomicAdd(&s_hist[pct_key(bits)], 1u);
                     }
                 }
             }
@@ -440,43 +462,42 @@ __global__ void k_bandpass_cols_generic(Batch b, const Geom* __restrict__ g, int
 }
 
 template <int R, typename T>
-static hipError_t launch_bp_t(const Batch& b, const K1Params& kp, hipStream_t s) {
-    hipLaunchKernelGGL((k_bandpass<R, T>), dim3(b.k1_items), dim3(64), 0, s, b, kp);
+static hipError_t launch_bp_t(const Batch& b, const K1Params& kp, bool fast, hipStream_t s) {
+    if (fast) hipLaunchKernelGGL((k_bandpass<R, T, true>), dim3(b.k1_items), dim3(64), 0, s, b, kp);
+    else hipLaunchKernelGGL((k_bandpass<R, T, false>), dim3(b.k1_items), dim3(64), 0, s, b, kp);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t launch_bp_r(int R, const Batch& b, const K1Params& kp, hipStream_t s) {
+static hipError_t launch_bp_r(int R, const Batch& b, const K1Params& kp, bool fast, hipStream_t s) {
     switch (R) {
-        case 0: return launch_bp_t<0, T>(b, kp, s);
-        case 1: return launch_bp_t<1, T>(b, kp, s);
-        case 2: return launch_bp_t<2, T>(b, kp, s);
-        case 3: return launch_bp_t<3, T>(b, kp, s);
-        case 4: return launch_bp_t<4, T>(b, kp, s);
-        case 5: return launch_bp_t<5, T>(b, kp, s);
-        case 6: return launch_bp_t<6, T>(b, kp, s);
-        case 7: return launch_bp_t<7, T>(b, kp, s);
+        case 1: return launch_bp_t<1, T>(b, kp, fast, s);
+        case 2: return launch_bp_t<2, T>(b, kp, fast, s);
+        case 3: return launch_bp_t<3, T>(b, kp, fast, s);
+        case 4: return launch_bp_t<4, T>(b, kp, fast, s);
+        case 5: return launch_bp_t<5, T>(b, kp, fast, s);
+        case 6: return launch_bp_t<6, T>(b, kp, fast, s);
+        case 7: return launch_bp_t<7, T>(b, kp, fast, s);
         default: return hipErrorInvalidValue;
     }
 }
 
+bool bandpass_fast_path(const Batch& b, const Geom& g) {
+    return g.R >= 1 && g.R <= 7 && b.vec && (b.in_type == IN_U16 || b.in_type == IN_U8);
+}
+
 hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, int seg_rows,
                            hipStream_t s) {
-    const int hist = b.hist1 != nullptr && g.has_pct;
-    if (g.R <= 7) {
-        K1Params kp;
-        for (int i = 0; i < 15; ++i) kp.w[i] = (i < 2 * g.R + 1) ? g.w[i] : 0.f;
-        kp.gn = g.gn; kp.rgn = g.rgn; kp.kk = g.kk; kp.rkk = g.rkk;
-        kp.fast_gn = g.fastdiv_gn; kp.fast_kk = g.fastdiv_kk;
-        kp.seg = seg_rows;
-        kp.hist = hist;
-        kp.vec = b.vec;
-        const int it = b.in_type;
-        if (it == IN_U16) return launch_bp_r<uint16_t>(g.R, b, kp, s);
-        if (it == IN_U8) return launch_bp_r<uint8_t>(g.R, b, kp, s);
-        return launch_bp_r<float>(g.R, b, kp, s);
-    }
-    return hipErrorInvalidValue;  // generic path is launched by the host with a temp
+    (void)d_geom;
+    if (!bandpass_fast_path(b, g)) return hipErrorInvalidValue;
+    K1Params kp;
+    for (int i = 0; i < 15; ++i) kp.w[i] = (i < 2 * g.R + 1) ? g.w[i] : 0.f;
+    kp.gn = g.gn; kp.rgn = g.rgn; kp.kk = g.kk; kp.rkk = g.rkk;
+    kp.seg = seg_rows;
+    kp.hist = b.hist1 != nullptr && g.has_pct;
+    const bool fast = g.fastdiv_gn && g.fastdiv_kk;
+    if (b.in_type == IN_U16) return launch_bp_r<uint16_t>(g.R, b, kp, fast, s);
+    return launch_bp_r<uint8_t>(g.R, b, kp, fast, s);
 }
 
 hipError_t launch_bandpass_generic(const Batch& b, const Geom* d_geom, uint32_t max_px,
@@ -598,6 +619,8 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     __shared__ __attribute__((aligned(16))) float RM[LR * TILE_W];
     __shared__ uint64_t s_mask[TILE_H];
     __shared__ uint32_t s_off[TILE_H];
+    __shared__ uint32_t s_lcnt;
+    if (threadIdx.x == 0) s_lcnt = 0;
 
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
@@ -610,13 +633,23 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     const int a = dp.a;
     const float* __restrict__ P = fr.P;
 
-    // stage P over the tile plus the window halo (-inf outside the frame)
-    for (int idx = tid; idx < LR * LC; idx += 256) {
+    // stage P over the tile plus the window halo (-inf outside the frame);
+    // every load is issued before the first LDS store
+    constexpr int NFILL = (LR * LC + 255) / 256;
+    float fill[NFILL];
+#pragma unroll
+    for (int q = 0; q < NFILL; ++q) {
+        const int idx = tid + q * 256;
         const int r = idx / LC, j = idx - r * LC;
         const int u = y0 - a + r, v = x0 - a + j;
-        float val = __uint_as_float(NEG_INF_BITS);
-        if (u >= 0 && u < H && v >= 0 && v < W) val = P[(size_t)u * W + v];
-        L[r * LS + j] = val;
+        fill[q] = __uint_as_float(NEG_INF_BITS);
+        if (idx < LR * LC && u >= 0 && u < H && v >= 0 && v < W) fill[q] = P[(size_t)u * W + v];
+    }
+#pragma unroll
+    for (int q = 0; q < NFILL; ++q) {
+        const int idx = tid + q * 256;
+        const int r = idx / LC, j = idx - r * LC;
+        if (idx < LR * LC) L[r * LS + j] = fill[q];
     }
     __syncthreads();
 
@@ -656,7 +689,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         bin_lo = b.selstate[f * 4 + 0];
         bin_hi = b.selstate[f * 4 + 2];
     }
-    uint32_t* list = b.list + fr.list_base;
+    uint32_t* list = b.list + (size_t)item * TILE_PX;
     uint32_t cbits = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
@@ -678,7 +711,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
             const uint64_t mm = __ballot(match);
             if (mm) {
                 uint32_t basei = 0;
-                if (c == 0) basei = atomicAdd(&b.list_count[f], (uint32_t)__popcll(mm));
+                if (c == 0) basei = atomicAdd(&s_lcnt, (uint32_t)__popcll(mm));
                 basei = __shfl(basei, 0, 64);
                 if (match) list[basei + __popcll(mm & lanemask_lt(c))] = bits;
             }
@@ -691,7 +724,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         s_off[tid] = inc - cnt;
         if (tid == 63) {
             b.tile_count[item] = inc;
-            if (inc) atomicAdd((unsigned long long*)b.cand_total, (unsigned long long)inc);
+            if (dp.has_pct) b.tile_lcount[item] = s_lcnt;
         }
     }
     __syncthreads();
@@ -712,6 +745,8 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
     extern __shared__ float dyn[];
     __shared__ uint64_t s_mask[TILE_H];
     __shared__ uint32_t s_off[TILE_H];
+    __shared__ uint32_t s_lcnt;
+    if (threadIdx.x == 0) s_lcnt = 0;
     const int KD = dp.kd;
     const int LR = TILE_H + max(KD, 1) - 1, LC = TILE_W + max(KD, 1) - 1;
     float* L = dyn;                  // LR * LC
@@ -748,7 +783,7 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
         bin_lo = b.selstate[f * 4 + 0];
         bin_hi = b.selstate[f * 4 + 2];
     }
-    uint32_t* list = b.list + fr.list_base;
+    uint32_t* list = b.list + (size_t)item * TILE_PX;
     uint32_t cbits = 0;
     for (int m = 0; m < 16; ++m) {
         const int r = k * 16 + m;
@@ -770,7 +805,7 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
             const uint64_t mm = __ballot(match);
             if (mm) {
                 uint32_t basei = 0;
-                if (c == 0) basei = atomicAdd(&b.list_count[f], (uint32_t)__popcll(mm));
+                if (c == 0) basei = atomicAdd(&s_lcnt, (uint32_t)__popcll(mm));
                 basei = __shfl(basei, 0, 64);
                 if (match) list[basei + __popcll(mm & lanemask_lt(c))] = bits;
             }
@@ -783,7 +818,7 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
         s_off[tid] = inc - cnt;
         if (tid == 63) {
             b.tile_count[item] = inc;
-            if (inc) atomicAdd((unsigned long long*)b.cand_total, (unsigned long long)inc);
+            if (dp.has_pct) b.tile_lcount[item] = s_lcnt;
         }
     }
     __syncthreads();
@@ -838,48 +873,47 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 // ---------------------------------------------------------------------------
 // select2: exact order statistics from the value list, then the threshold
 // ---------------------------------------------------------------------------
-__device__ uint32_t select_in_list(const uint32_t* list, uint32_t cnt, int bin, int rank,
+// Radix selection of the `rank`-th smallest value of histogram bin `bin`
+// among the per-tile value lists of one frame (bits [18:8], then [7:0]).
+// Each wave walks whole tiles; lanes stride over the tile's list.
+__device__ uint32_t select_in_list(const Batch& b, const Frame& fr, int bin, int rank,
                                    uint32_t* s_hist /* >= 2048 */, uint32_t* s_res) {
     const int t = threadIdx.x, nt = blockDim.x;
-    // level A: bits [18:8]
-    for (int i = t; i < 2048; i += nt) s_hist[i] = 0;
-    __syncthreads();
-    for (uint32_t i = t; i < cnt; i += nt) {
-        const uint32_t bits = list[i];
-        if ((int)pct_key(bits) == bin) atomicAdd(&s_hist[(bits >> 8) & 2047u], 1u);
-    }
-    __syncthreads();
-    if (t == 0) {
-        uint32_t cum = 0;
-        int sa = 0;
-        for (int i = 0; i < 2048; ++i) {
-            if ((uint32_t)rank < cum + s_hist[i]) { sa = i; break; }
-            cum += s_hist[i];
+    const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
+    const uint32_t ntiles = fr.tx * fr.ty;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nb = pass == 0 ? 2048 : 256;
+        for (int i = t; i < nb; i += nt) s_hist[i] = 0;
+        __syncthreads();
+        const uint32_t sa = pass == 0 ? 0u : s_res[0];
+        for (uint32_t tl = wave; tl < ntiles; tl += nw) {
+            const uint32_t it = fr.tile_base + tl;
+            const uint32_t cnt = b.tile_lcount[it];
+            const uint32_t* list = b.list + (size_t)it * TILE_PX;
+            for (uint32_t i = lane; i < cnt; i += 64) {
+                const uint32_t bits = list[i];
+                if ((int)pct_key(bits) != bin) continue;
+                if (pass == 0) atomicAdd(&s_hist[(bits >> 8) & 2047u], 1u);
+                else if (((bits >> 8) & 2047u) == sa) atomicAdd(&s_hist[bits & 255u], 1u);
+            }
         }
-        s_res[0] = (uint32_t)sa;
-        s_res[1] = (uint32_t)rank - cum;
-    }
-    __syncthreads();
-    const uint32_t sa = s_res[0], ra = s_res[1];
-    // level B: bits [7:0]
-    for (int i = t; i < 256; i += nt) s_hist[i] = 0;
-    __syncthreads();
-    for (uint32_t i = t; i < cnt; i += nt) {
-        const uint32_t bits = list[i];
-        if ((int)pct_key(bits) == bin && ((bits >> 8) & 2047u) == sa)
-            atomicAdd(&s_hist[bits & 255u], 1u);
-    }
-    __syncthreads();
-    if (t == 0) {
-        uint32_t cum = 0;
-        uint32_t sb = 0;
-        for (int i = 0; i < 256; ++i) {
-            if (ra < cum + s_hist[i]) { sb = (uint32_t)i; break; }
-            cum += s_hist[i];
+        __syncthreads();
+        if (t == 0) {
+            const uint32_t want = pass == 0 ? (uint32_t)rank : s_res[1];
+            uint32_t cum = 0, sel = 0;
+            for (int i = 0; i < nb; ++i) {
+                if (want < cum + s_hist[i]) { sel = (uint32_t)i; break; }
+                cum += s_hist[i];
+            }
+            if (pass == 0) {
+                s_res[0] = sel;
+                s_res[1] = want - cum;
+            } else {
+                s_res[2] = ((uint32_t)(bin - 1) << 19) | (s_res[0] << 8) | sel;
+            }
         }
-        s_res[2] = ((uint32_t)(bin - 1) << 19) | (sa << 8) | sb;
+        __syncthreads();
     }
-    __syncthreads();
     const uint32_t v = s_res[2];
     __syncthreads();
     return v;
@@ -896,12 +930,10 @@ __global__ __launch_bounds__(1024) void k_select2(Batch b, float percentile) {
     pct_ranks(n, percentile, lo, hi, frac);
     const int bin_lo = b.selstate[f * 4 + 0], r_lo = b.selstate[f * 4 + 1];
     const int bin_hi = b.selstate[f * 4 + 2], r_hi = b.selstate[f * 4 + 3];
-    const uint32_t* list = b.list + fr.list_base;
-    const uint32_t cnt = b.list_count[f];
     float vlo = 0.f, vhi = 0.f;
-    if (bin_lo != 0) vlo = __uint_as_float(select_in_list(list, cnt, bin_lo, r_lo, s_hist, s_res));
+    if (bin_lo != 0) vlo = __uint_as_float(select_in_list(b, fr, bin_lo, r_lo, s_hist, s_res));
     if (hi == lo) vhi = vlo;
-    else if (bin_hi != 0) vhi = __uint_as_float(select_in_list(list, cnt, bin_hi, r_hi, s_hist, s_res));
+    else if (bin_hi != 0) vhi = __uint_as_float(select_in_list(b, fr, bin_hi, r_hi, s_hist, s_res));
     if (threadIdx.x == 0) b.thr[f] = (double)vlo + frac * ((double)vhi - (double)vlo);
 }
 
@@ -1128,20 +1160,21 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 }
 
 // ---------------------------------------------------------------------------
-// K4 dedup: one wave per tile; neighbours are read tile by tile with
-// wave-uniform loads.  The last wave of a frame scans its tiles' keep counts.
+// K4 dedup: one wave per tile.  The records of the tiles that can hold a
+// neighbour (start pixels within sep + 2*(max_iterations + rint) per axis) are
+// staged in LDS with parallel loads, then every own candidate scans them.
+// Neighbourhoods larger than the staging area fall back to reading the
+// records from global memory.  The last wave of a frame scans its tiles'
+// keep counts into row offsets.
 // ---------------------------------------------------------------------------
+constexpr int DD_CAP = 512;   // staged records per wave
+
 __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__ g) {
+    __shared__ float s_m[4][DD_CAP], s_y[4][DD_CAP], s_x[4][DD_CAP];
+    __shared__ int32_t s_i[4][DD_CAP];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
-    if (item >= b.tiles) return;
-    const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
-    const Frame fr = b.frames[f];
-    const uint32_t local = item - fr.tile_base;
-    const int W = (int)fr.W;
-    const int tx = (int)(local % fr.tx), ty = (int)(local / fr.tx);
-    const uint32_t cnt = b.tile_count[item];
-    const size_t slot0 = (size_t)item * TILE_PX;
+    const bool active = item < b.tiles;
     const uint64_t cap = b.slot_cap;
     const float* rmass = b.rec + REC_MASS * cap;
     const float* ry = b.rec + REC_Y * cap;
@@ -1149,74 +1182,119 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     const float rad = g->sep;
     const float r2 = rad * rad;
     const int rty = g->dedup_rty, rtx = g->dedup_rtx;
-    uint32_t kept = 0;
 
-    for (uint32_t ib = 0; ib < cnt; ib += 64) {
-        const uint32_t i = ib + lane;
-        const bool own = i < cnt;
-        float m = 0.f, y = 0.f, x = 0.f;
-        int64_t idx = 0;
-        if (own) {
-            m = rmass[slot0 + i];
-            y = ry[slot0 + i];
-            x = rx[slot0 + i];
-            const uint32_t l = b.slots[slot0 + i];
-            idx = (int64_t)(ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
+    uint32_t f = 0, cnt = 0;
+    Frame fr;
+    int tx = 0, ty = 0, W = 0, y_lo = 0, y_hi = -1, x_lo = 0, x_hi = -1;
+    if (active) {
+        f = find_frame(b.tile_bases, b.nframes, item);
+        fr = b.frames[f];
+        const uint32_t local = item - fr.tile_base;
+        W = (int)fr.W;
+        tx = (int)(local % fr.tx);
+        ty = (int)(local / fr.tx);
+        cnt = b.tile_count[item];
+        y_lo = max(0, ty - rty);
+        y_hi = min((int)fr.ty - 1, ty + rty);
+        x_lo = max(0, tx - rtx);
+        x_hi = min((int)fr.tx - 1, tx + rtx);
+    }
+    const int nxr = x_hi - x_lo + 1;
+    const int ntl = active ? (y_hi - y_lo + 1) * nxr : 0;
+    bool staged = active && ntl <= 64;
+    uint32_t ncnt = 0, it2 = 0;
+    if (staged && lane < ntl) {
+        const int ny = y_lo + lane / nxr, nx = x_lo + lane % nxr;
+        it2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+        ncnt = b.tile_count[it2];
+    }
+    const uint32_t incl = wave_incl_scan(ncnt, lane);
+    const uint32_t excl = incl - ncnt;
+    const uint32_t M = __shfl(incl, 63, 64);
+    staged = staged && M <= (uint32_t)DD_CAP;
+    if (staged) {
+        // every lane stays in the loop so that the shuffles read live lanes
+        for (uint32_t rb = 0; rb < M; rb += 64) {
+            const uint32_t r = rb + lane;
+            int j = 0;
+            for (int q = 1; q < ntl; ++q)
+                if (__shfl(excl, q, 64) <= r) j = q;
+            const uint32_t t2 = __shfl(it2, j, 64);
+            const uint32_t e2 = __shfl(excl, j, 64);
+            if (r >= M) continue;
+            const size_t s = (size_t)t2 * TILE_PX + (r - e2);
+            const uint32_t l2 = b.slots[s];
+            const uint32_t loc2 = t2 - fr.tile_base;
+            const int py = (int)(loc2 / fr.tx) * TILE_H + (int)(l2 / TILE_W);
+            const int px = (int)(loc2 % fr.tx) * TILE_W + (int)(l2 % TILE_W);
+            s_m[wave][r] = rmass[s];
+            s_y[wave][r] = ry[s];
+            s_x[wave][r] = rx[s];
+            s_i[wave][r] = py * W + px;
         }
-        bool keep = own && (m != 0.f);
-        if (__ballot(keep)) {
-            for (int ny = max(0, ty - rty); ny <= min((int)fr.ty - 1, ty + rty); ++ny) {
-                for (int nx = max(0, tx - rtx); nx <= min((int)fr.tx - 1, tx + rtx); ++nx) {
-                    const uint32_t it2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-                    const uint32_t c2 = b.tile_count[it2];
-                    const size_t s2 = (size_t)it2 * TILE_PX;
-                    for (uint32_t j = 0; j < c2; ++j) {
-                        const float nm = rmass[s2 + j];
+    }
+    __syncthreads();
+
+    uint32_t kept = 0;
+    if (active) {
+        const size_t slot0 = (size_t)item * TILE_PX;
+        for (uint32_t ib = 0; ib < cnt; ib += 64) {
+            const uint32_t i = ib + lane;
+            const bool own = i < cnt;
+            float m = 0.f, y = 0.f, x = 0.f;
+            int32_t idx = -1;
+            if (own) {
+                m = rmass[slot0 + i];
+                y = ry[slot0 + i];
+                x = rx[slot0 + i];
+                const uint32_t l = b.slots[slot0 + i];
+                idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
+            }
+            bool keep = own && (m != 0.f);
+            if (__ballot(keep)) {
+                if (staged) {
+                    for (uint32_t j = 0; j < M; ++j) {
+                        const float nm = s_m[wave][j];
                         if (nm == 0.f) continue;
-                        const float nyv = ry[s2 + j], nxv = rx[s2 + j];
-                        const float dy = nyv - y, dx = nxv - x;
+                        const float dy = s_y[wave][j] - y, dx = s_x[wave][j] - x;
                         float d = dy * dy;
                         d = d + dx * dx;
-                        if (d < r2 && !(it2 == item && j == i)) {
-                            bool drop = m < nm;
-                            if (!drop && m == nm) {
-                                const uint32_t l2 = b.slots[s2 + j];
-                                const int64_t nidx =
-                                    (int64_t)(ny * TILE_H + (int)(l2 / TILE_W)) * W +
-                                    (nx * TILE_W + (int)(l2 % TILE_W));
-                                drop = idx > nidx;
+                        const int32_t nidx = s_i[wave][j];
+                        if (d < r2 && nidx != idx && (m < nm || (m == nm && idx > nidx)))
+                            keep = false;
+                    }
+                } else {
+                    for (int ny = y_lo; ny <= y_hi; ++ny) {
+                        for (int nx = x_lo; nx <= x_hi; ++nx) {
+                            const uint32_t t2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+                            const uint32_t c2 = b.tile_count[t2];
+                            const size_t s2 = (size_t)t2 * TILE_PX;
+                            for (uint32_t j = 0; j < c2; ++j) {
+                                const float nm = rmass[s2 + j];
+                                if (nm == 0.f) continue;
+                                const float dy = ry[s2 + j] - y, dx = rx[s2 + j] - x;
+                                float d = dy * dy;
+                                d = d + dx * dx;
+                                if (d < r2 && !(t2 == item && j == i)) {
+                                    bool drop = m < nm;
+                                    if (!drop && m == nm) {
+                                        const uint32_t l2 = b.slots[s2 + j];
+                                        const int32_t nidx =
+                                            (ny * TILE_H + (int)(l2 / TILE_W)) * W +
+                                            (nx * TILE_W + (int)(l2 % TILE_W));
+                                        drop = idx > nidx;
+                                    }
+                                    if (drop) keep = false;
+                                }
                             }
-                            if (drop) keep = false;
                         }
                     }
                 }
             }
+            if (own) b.keep[slot0 + i] = keep ? 1 : 0;
+            kept += (uint32_t)__popcll(__ballot(keep));
         }
-        if (own) b.keep[slot0 + i] = keep ? 1 : 0;
-        kept += (uint32_t)__popcll(__ballot(keep));
-    }
-    if (lane == 0) b.tile_keep[item] = kept;
-
-    // last wave of this frame: exclusive scan of the frame's tile keep counts
-    __threadfence();
-    uint32_t old = 0;
-    if (lane == 0) old = atomicAdd(&b.done[f], 1u);
-    old = __shfl(old, 0, 64);
-    const uint32_t ntiles = fr.tx * fr.ty;
-    if (old == ntiles - 1) {
-        __threadfence();
-        uint32_t carry = 0;
-        for (uint32_t base = 0; base < ntiles; base += 64) {
-            const uint32_t k = base + lane;
-            const uint32_t c =
-                k < ntiles ? __hip_atomic_load(&b.tile_keep[fr.tile_base + k], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT)
-                           : 0u;
-            const uint32_t inc = wave_incl_scan(c, lane);
-            if (k < ntiles) b.tile_row_off[fr.tile_base + k] = carry + inc - c;
-            carry += __shfl(inc, 63, 64);
-        }
-        if (lane == 0) b.frame_rows[f] = carry;
+        if (lane == 0) b.tile_keep[item] = kept;
     }
 }
 
@@ -1227,19 +1305,63 @@ hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom, hipSt
 }
 
 // ---------------------------------------------------------------------------
+// tile scan: one workgroup per frame turns its tiles' keep counts into row
+// offsets and sums its candidates and rows
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tile_scan(Batch b) {
+    __shared__ uint32_t s_part[4];
+    __shared__ uint32_t s_cand[4];
+    const uint32_t f = blockIdx.x;
+    const Frame fr = b.frames[f];
+    const uint32_t ntiles = fr.tx * fr.ty;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t carry = 0, cands = 0;
+    for (uint32_t base = 0; base < ntiles; base += 256) {
+        const uint32_t k = base + t;
+        const uint32_t c = k < ntiles ? b.tile_keep[fr.tile_base + k] : 0u;
+        cands += k < ntiles ? b.tile_count[fr.tile_base + k] : 0u;
+        const uint32_t inc = wave_incl_scan(c, lane);
+        if (lane == 63) s_part[wave] = inc;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int w = 0; w < wave; ++w) before += s_part[w];
+        if (k < ntiles) b.tile_row_off[fr.tile_base + k] = before + inc - c;
+        carry += s_part[0] + s_part[1] + s_part[2] + s_part[3];
+        __syncthreads();
+    }
+    cands = wave_sum_u(cands);
+    if (lane == 0) s_cand[wave] = cands;
+    __syncthreads();
+    if (t == 0) {
+        b.frame_rows[f] = carry;
+        b.frame_cands[f] = s_cand[0] + s_cand[1] + s_cand[2] + s_cand[3];
+    }
+}
+
+hipError_t launch_tile_scan(const Batch& b, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_scan, dim3(b.nframes), dim3(256), 0, s, b);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // fscan: frame row bases (continuing the running count across batches)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_frame_scan(Batch b) {
     __shared__ uint64_t s_part[256];
+    __shared__ uint64_t s_cand[256];
     const int t = threadIdx.x;
     const uint32_t n = b.nframes;
     const uint32_t per = (n + 255) / 256;
-    uint64_t sum = 0;
+    uint64_t sum = 0, cs = 0;
     for (uint32_t i = 0; i < per; ++i) {
         const uint32_t k = t * per + i;
-        if (k < n) sum += b.frame_rows[k];
+        if (k < n) {
+            sum += b.frame_rows[k];
+            cs += b.frame_cands[k];
+        }
     }
     s_part[t] = sum;
+    s_cand[t] = cs;
     __syncthreads();
     uint64_t before = *b.row_carry;
     for (int i = 0; i < t; ++i) before += s_part[i];
@@ -1251,7 +1373,12 @@ __global__ __launch_bounds__(256) void k_frame_scan(Batch b) {
         }
     }
     __syncthreads();
-    if (t == 255) *b.row_carry = before;
+    if (t == 255) {
+        *b.row_carry = before;
+        uint64_t c = 0;
+        for (int i = 0; i < 256; ++i) c += s_cand[i];
+        *b.cand_total += c;
+    }
 }
 
 hipError_t launch_frame_scan(const Batch& b, hipStream_t s) {
