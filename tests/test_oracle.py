@@ -1,0 +1,307 @@
+"""Pins the CPU oracle (oracle/gt_oracle.c) before it is trusted as the checker.
+
+  * golden vectors from the reference's own GPU dumps (tests/golden):
+    F1 grey_lion -> masses.bin (bandpass + disk mass + border band),
+    F2 processed frame -> centers.bin (centroid offset convention);
+  * special cases that reduce to library routines (scipy.ndimage filters,
+    numpy.quantile), in binary64;
+  * brute force on small inputs (deduplication) and closed forms
+    (symmetric spot, constant frame, stale-offset walk).
+Each check is chosen so that a plausible mistake (a wrong boundary rule, an
+off-by-one window, a swapped axis, a transitive dedup) fails it.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.ndimage as nd
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def lion():
+    return np.load(os.path.join(GOLD, "grey_lion_u8.npy")).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def lion_crops():
+    return dict(np.load(os.path.join(GOLD, "grey_lion_masses_crops.npz")))
+
+
+# ---------------------------------------------------------------- geometry --
+
+@pytest.mark.parametrize("sep,kd,a", [(9, 12, 5), (11, 15, 7), (10, 14, 6), (5, 7, 3), (0, 0, 0)])
+def test_dilation_geometry(sep, kd, a):
+    """gpu_setup.rs:48 (2*sep/sqrt2 as u32) and max_rows.wgsl:17 window start."""
+    g = O.geometry(O.Params(diameter=9, separation=sep))
+    assert (g.kd, g.a) == (kd, a)
+
+
+def test_disk_and_band_geometry():
+    g = O.geometry(O.Params(diameter=10))
+    assert (g.band, g.rint) == (5, 4)   # even diameter: band d/2, disk (d-1)/2
+    g = O.geometry(O.Params(diameter=11))
+    assert (g.band, g.rint) == (5, 5)
+
+
+# ------------------------------------------------------------ golden F1/F2 --
+
+def test_f1_masses_match_reference_gpu_dump(lion, lion_crops):
+    P = O.bandpass(lion, 9, 1.0)
+    m, _ = O.masses_centers(P, 9)
+    rel = []
+    for name in ("tl", "br", "mid"):
+        r0, r1, c0, c1 = lion_crops[name + "_box"]
+        ref = lion_crops[name]
+        got = m[r0:r1, c0:c1]
+        assert np.array_equal(got != 0, ref != 0), name
+        # the reference's GPU exp()/FMA differ from the oracle in the last bits
+        assert np.abs(got - ref).max() <= 3e-3, name
+        big = ref > 100
+        rel.append(np.abs(got - ref)[big] / ref[big])
+    rel = np.concatenate(rel)
+    assert len(rel) > 100 and rel.max() <= 1e-5
+    nz = np.nonzero(m)
+    bbox = [nz[0].min(), nz[0].max(), nz[1].min(), nz[1].max()]
+    assert bbox == list(lion_crops["nonzero_bbox"])
+    assert len(nz[0]) == int(lion_crops["nonzero_count"][0])
+
+
+def _bandpass_scipy(img, k, noise, gauss_mode="constant", box_mode="reflect"):
+    """binary64 restatement with scipy filters (Gaussian normalised by the sum
+    of all taps, boxcar summed over k+1 taps for even k, divided by k*k)."""
+    R = k // 2
+    x = np.arange(-R, R + 1, dtype=np.float64)
+    w = np.exp(-x * x / (2 * noise))
+    w /= w.sum()
+    img = img.astype(np.float64)
+    g = nd.correlate1d(nd.correlate1d(img, w, axis=1, mode=gauss_mode), w, axis=0, mode=gauss_mode)
+    box = nd.uniform_filter(img, size=2 * R + 1, mode=box_mode) * (2 * R + 1) ** 2 / (k * k)
+    return np.maximum(g - box, 0.0)
+
+
+def test_f1_discriminates_boundary_rules(lion, lion_crops):
+    """The dump fixes Q3/Q4: the true rules match it to 3e-3, swapping either
+    boundary rule misses the border crops by more than 1."""
+    P = _bandpass_scipy(lion, 9, 1.0).astype(np.float32)
+    m, _ = O.masses_centers(P, 9)
+    for name in ("tl", "br"):
+        r0, r1, c0, c1 = lion_crops[name + "_box"]
+        assert np.abs(m[r0:r1, c0:c1] - lion_crops[name]).max() < 3e-3
+    for gm, bm in (("reflect", "reflect"), ("constant", "nearest")):
+        P = _bandpass_scipy(lion, 9, 1.0, gm, bm).astype(np.float32)
+        m, _ = O.masses_centers(P, 9)
+        err = max(np.abs(m[b[0]:b[1], b[2]:b[3]] - lion_crops[n]).max()
+                  for n, b in (("tl", lion_crops["tl_box"]), ("br", lion_crops["br_box"])))
+        assert err > 1.0, (gm, bm)
+
+
+def test_f2_centroid_offsets_match_reference_dump():
+    d = np.load(os.path.join(GOLD, "processed_centers_crop.npz"))
+    proc, cent = d["processed"], d["centers"]
+    p0 = d["proc_box"][0]
+    q0, q1 = d["cent_box"][0], d["cent_box"][1]
+    m, c = O.masses_centers(proc, 9)
+    s = slice(q0 - p0, q1 - p0)
+    got, mm = c[s, s], m[s, s]
+    ok = np.isfinite(cent).all(-1) & (mm > 1e-2)
+    assert ok.mean() > 0.9
+    assert np.abs(got[ok] - cent[ok]).max() <= 1e-3
+    # a transposed (col, row) convention would be far off
+    assert np.abs(got[ok][:, ::-1] - cent[ok]).max() > 0.1
+
+
+# ------------------------------------------------------ library special cases --
+
+@pytest.mark.parametrize("shape", [(64, 80), (31, 17), (9, 12)])
+@pytest.mark.parametrize("k,noise", [(9, 1.0), (11, 1.0), (7, 2.0), (10, 1.0), (5, 0.5)])
+def test_bandpass_f64_equals_scipy(shape, k, noise):
+    rng = np.random.default_rng(hash((shape, k)) & 0xffff)
+    if min(shape) < k // 2:
+        pytest.skip("frame smaller than the filter half-width")
+    img = rng.poisson(200, shape).astype(np.float64)
+    got = O.bandpass(img, k, noise, f64=True)
+    ref = _bandpass_scipy(img, k, noise)
+    assert np.abs(got - ref).max() <= 1e-9 * max(1.0, np.abs(ref).max())
+    got32 = O.bandpass(img.astype(np.float32), k, noise)
+    assert np.abs(got32 - ref).max() <= 2e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_bandpass_constant_frame_is_zero_inside():
+    img = np.full((40, 50), 123.0, np.float32)
+    P = O.bandpass(img, 9, 1.0)
+    assert np.all(P[4:-4, 4:-4] == 0)
+    # near the edges the zero-padded Gaussian loses mass: result stays <= 0
+    assert np.all(P >= 0)
+
+
+@pytest.mark.parametrize("sep", [3, 5, 9, 10, 11])
+def test_is_max_equals_maximum_filter(sep):
+    rng = np.random.default_rng(sep)
+    P = rng.integers(0, 6, (48, 57)).astype(np.float32)   # many ties / plateaus
+    g = O.geometry(O.Params(diameter=3, separation=sep))
+    ref = P == nd.maximum_filter(P, size=g.kd, origin=g.a - g.kd // 2, mode="constant",
+                                 cval=-np.inf)
+    assert np.array_equal(O.is_max_map(P, sep), ref)
+    # a symmetric (off-by-one) window differs for even kd on tie-free data
+    if g.kd % 2 == 0:
+        Q = rng.random((48, 57)).astype(np.float32)
+        alt = Q == nd.maximum_filter(Q, size=g.kd, mode="constant", cval=-np.inf)
+        assert not np.array_equal(O.is_max_map(Q, sep), alt)
+
+
+@pytest.mark.parametrize("pct", [0.0, 10.0, 50.0, 64.0, 99.5, 100.0])
+def test_percentile_equals_numpy_quantile(pct):
+    rng = np.random.default_rng(int(pct * 10))
+    P = np.maximum(rng.normal(0, 1, 5001), 0).astype(np.float32)   # ~half exact zeros
+    P[:100] = 0.25                                                   # ties
+    got = O.percentile_threshold(P, pct)
+    ref = np.quantile(P.astype(np.float64), pct / 100.0, method="linear")
+    assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+# -------------------------------------------------------------- brute force --
+
+def _brute_dedup(y, x, m, idx, sep):
+    keep = np.ones(len(y), bool)
+    r2 = np.float32(sep) * np.float32(sep)
+    for i in range(len(y)):
+        for j in range(len(y)):
+            if i == j:
+                continue
+            dy = np.float32(y[j] - y[i])
+            dx = np.float32(x[j] - x[i])
+            d = np.float32(np.float32(dy * dy) + np.float32(dx * dx))
+            if d < r2 and (m[i] < m[j] or (m[i] == m[j] and idx[i] > idx[j])):
+                keep[i] = False
+                break
+    return keep
+
+
+def test_dedup_semantics_small():
+    # A (heaviest) drops B; B still drops C (single pass, not transitive)
+    y = np.array([0, 0, 0, 50], np.float32)
+    x = np.array([0, 8, 16, 50], np.float32)
+    m = np.array([10, 5, 1, 3], np.float32)
+    idx = np.arange(4)
+    keep = O.dedup_bruteforce(y, x, m, idx, 9)
+    assert keep.tolist() == [True, False, False, True]
+    assert keep.tolist() == _brute_dedup(y, x, m, idx, 9).tolist()
+    # equal masses: the lower index wins; distance exactly `separation` is not a neighbour
+    y = np.array([0, 0, 0], np.float32)
+    x = np.array([0, 3, 12], np.float32)
+    m = np.array([2, 2, 2], np.float32)
+    assert O.dedup_bruteforce(y, x, m, np.array([5, 1, 9]), 9).tolist() == [False, True, True]
+
+
+def test_locate_dedup_equals_bruteforce():
+    """The oracle's grid deduplication against an O(n^2) pass."""
+    rng = np.random.default_rng(7)
+    img = rng.poisson(120, (160, 170)).astype(np.float32)
+    p = O.Params(diameter=7, separation=6, smoothing_size=7)
+    cand, y, x, m = O.candidates_frame(img, p)
+    assert len(cand) > 50
+    keep = _brute_dedup(y, x, m, cand, 6)
+    rows, rcand, _ = O.locate_frame(img, p)
+    assert np.array_equal(rcand, cand[keep])
+    assert np.array_equal(rows[:, 1].astype(np.float32), m[keep])
+
+
+def _spot(shape, cy, cx, amp, sigma, bg=10.0):
+    yy, xx = np.mgrid[0:shape[0], 0:shape[1]]
+    return (bg + amp * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * sigma ** 2))).astype(np.float32)
+
+
+def test_symmetric_spot_on_pixel_centre():
+    img = _spot((64, 64), 32, 30, 1000.0, 1.5)
+    rows, cand, _ = O.locate_frame(img, O.Params(diameter=9, separation=9))
+    assert len(rows) == 1 and cand[0] == 32 * 64 + 30
+    assert abs(rows[0, 2] - 32) < 1e-5 and abs(rows[0, 3] - 30) < 1e-5
+
+
+@pytest.mark.parametrize("cy,cx", [(30.3, 31.7), (29.55, 33.45), (32.0, 28.2)])
+def test_subpixel_spot_position(cy, cx):
+    img = _spot((64, 64), cy, cx, 2000.0, 1.5)
+    rows, _, _ = O.locate_frame(img, O.Params(diameter=9, separation=9))
+    assert len(rows) == 1
+    assert abs(rows[0, 2] - cy) < 0.1 and abs(rows[0, 3] - cx) < 0.1
+
+
+def test_walk_stale_offset_after_last_move():
+    """walk_cols.wgsl:46-73: when the iteration cap is hit right after a move,
+    the offsets (and mass) of the previously evaluated pixel are used."""
+    # a broad plateau with a peak 3 px away: the local maximum of the coarse
+    # window sits on the plateau edge, the centroid pulls towards the peak
+    rng = np.random.default_rng(3)
+    img = _spot((64, 64), 32.0, 32.0, 3000.0, 2.5) + rng.normal(0, 0.01, (64, 64)).astype(np.float32)
+    p1 = O.Params(diameter=9, separation=3, max_iterations=1, literal=True)
+    p10 = O.Params(diameter=9, separation=3, max_iterations=10, literal=True)
+    P = O.bandpass(img, 9, 1.0)
+    mass, cen = O.masses_centers(P, 9)
+    cand, y1, x1, m1 = O.candidates_frame(img, p1)
+    moved = 0
+    for c, yy, xx, mm in zip(cand, y1, x1, m1):
+        u, v = divmod(int(c), 64)
+        dy = 1 if cen[u, v, 0] > 0.6 else (-1 if cen[u, v, 0] < -0.6 else 0)
+        dx = 1 if cen[u, v, 1] > 0.6 else (-1 if cen[u, v, 1] < -0.6 else 0)
+        assert yy == np.float32(u + dy) + cen[u, v, 0]
+        assert xx == np.float32(v + dx) + cen[u, v, 1]
+        assert mm == mass[u, v]
+        moved += (dy != 0) or (dx != 0)
+    assert moved > 0
+    # with enough iterations every walk ends on a pixel with |offset| <= 0.6
+    cand, y10, x10, m10 = O.candidates_frame(img, p10)
+    for yy, xx in zip(y10, x10):
+        u, v = int(round(yy)), int(round(xx))
+        assert abs(yy - u) <= 0.6 + 1e-6 and abs(xx - v) <= 0.6 + 1e-6
+
+
+def test_max_iterations_zero_gives_no_rows():
+    """With no walk the shader's zero-initialised state reads pixel 0, which
+    lies in the border band (mass 0), so nothing survives."""
+    img = _spot((64, 64), 32, 32, 1000.0, 1.5)
+    rows, _, _ = O.locate_frame(img, O.Params(diameter=9, separation=9, max_iterations=0))
+    assert len(rows) == 0
+
+
+def test_plateau_ties_are_all_candidates():
+    """walk_cols.wgsl:29: only a strictly greater neighbour rejects, so every
+    pixel of a flat top is a local maximum."""
+    P = np.zeros((40, 40), np.float32)
+    P[18:21, 18:22] = 5.0
+    P[30, 30] = 1.0
+    ism = O.is_max_map(P, 5)
+    assert ism[18:21, 18:22].all() and ism[30, 30]
+    assert ism.sum() > 12 + 1   # zero plateaus away from both blocks are maxima too
+
+
+def test_exact_shortcuts_do_not_change_rows():
+    rng = np.random.default_rng(11)
+    img = rng.poisson(100, (96, 112)).astype(np.float32)
+    img[40:44, 50:53] += 900
+    for kw in (dict(diameter=9, separation=9), dict(diameter=9, separation=9, percentile=64),
+               dict(diameter=7, separation=12, smoothing_size=9, minmass=-5.0)):
+        a = O.locate_frame(img, O.Params(literal=True, **kw))
+        b = O.locate_frame(img, O.Params(literal=False, **kw))
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_binary32_error_against_binary64():
+    """States the binary32 error of each output on a noisy frame."""
+    rng = np.random.default_rng(5)
+    img = rng.poisson(150, (256, 256)).astype(np.float32)
+    for (cy, cx) in rng.uniform(20, 236, (30, 2)):
+        img += _spot((256, 256), cy, cx, 1500.0, 2.0, bg=0.0)
+    p = O.Params(diameter=11, separation=11, percentile=64, cpu_processed=True, characterize=True)
+    r32, c32, t32 = O.locate_frame(img, p)
+    r64, c64, t64 = O.locate_frame(img, p, f64=True)
+    common, i32, i64 = np.intersect1d(c32, c64, return_indices=True)
+    assert len(common) >= 0.99 * max(len(c32), len(c64))
+    pos = np.abs(r32[i32, 2:4] - r64[i64, 2:4]).max()
+    mass = (np.abs(r32[i32, 1] - r64[i64, 1]) / r64[i64, 1]).max()
+    assert pos < 1e-3, pos
+    assert mass < 1e-5, mass
+    assert abs(t32 - t64) <= 1e-5 * max(1.0, abs(t64))

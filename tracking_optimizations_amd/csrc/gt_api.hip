@@ -530,6 +530,11 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         bases_off[bi] = bases_all.size();
         bases_all.insert(bases_all.end(), k1v.begin(), k1v.end());
         bases_all.insert(bases_all.end(), tbv.begin(), tbv.end());
+        // frame of every bandpass item and every detect tile (batch-relative)
+        for (uint32_t fi = 0; fi + 1 < (uint32_t)k1v.size(); ++fi)
+            for (uint32_t i = k1v[fi]; i < k1v[fi + 1]; ++i) bases_all.push_back(fi);
+        for (uint32_t fi = 0; fi + 1 < (uint32_t)tbv.size(); ++fi)
+            for (uint32_t i = tbv[fi]; i < tbv[fi + 1]; ++i) bases_all.push_back(fi);
     }
     HIP_TRY(ctx->frames_tab.ensure(ftab.size() * sizeof(Frame)));
     HIP_TRY(ctx->bases.ensure(bases_all.size() * 4));
@@ -562,6 +567,8 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             b.nframes = nb;
             b.k1_items = bases_all[bases_off[bi] + nb];
             b.tiles = bases_all[bases_off[bi] + nb + 1 + nb];
+            b.k1_frame = b.tile_bases + nb + 1;
+            b.tile_frame = b.k1_frame + b.k1_items;
             b.in_type = in_type;
             b.vec = vec ? 1 : 0;
             char* t3 = ctx->tile3.as<char>();

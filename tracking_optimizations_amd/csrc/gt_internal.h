@@ -77,6 +77,8 @@ struct Batch {
     const Frame* frames;   // device array [nframes]
     const uint32_t* k1_bases;   // [nframes + 1]
     const uint32_t* tile_bases; // [nframes + 1]
+    const uint32_t* k1_frame;   // [k1_items] frame of each bandpass item
+    const uint32_t* tile_frame; // [tiles] frame of each detect tile
     uint32_t nframes;
     uint32_t k1_items;
     uint32_t tiles;

@@ -44,15 +44,6 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t n) {
     return xcd * q + min(xcd, r) + local;
 }
 
-__device__ __forceinline__ uint32_t find_frame(const uint32_t* bases, uint32_t nframes,
-                                               uint32_t item) {
-    uint32_t lo = 0, hi = nframes;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (bases[mid] <= item) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ float cdiv(float x, float d, float rd, int fast) {
     if (fast) {
@@ -230,7 +221,7 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     __shared__ uint32_t s_hist[HIST_BINS];
 
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t f = find_frame(b.k1_bases, b.nframes, item);
+    const uint32_t f = b.k1_frame[item];
     const Frame fr = b.frames[f];
     const uint32_t local = item - fr.k1_base;
     const int H = (int)fr.H, W = (int)fr.W;
@@ -607,6 +598,11 @@ __device__ __forceinline__ void sliding_max(const float (&x)[N + K - 1], float (
     for (int i = 0; i < N; ++i) y[i] = fmaxf(S[i], Pm[i + K - 1]);
 }
 
+// One workgroup per 64x64 tile.  The tile and its window halo are staged in
+// LDS (all loads issued before the first store); the row maxima overwrite the
+// staged tile in place, so one tile needs only (64+KD-1) x (64+KD+3) floats of
+// LDS and about six workgroups share a CU, overlapping each other's load
+// latency.
 template <int KD>
 __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int N = 16;
@@ -615,97 +611,116 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int LS = ((LC + 3) / 4) * 4 + 4;
     constexpr int NIN = N + KD - 1;
     constexpr int NV4 = (NIN + 3) / 4;
+    constexpr int NFILL = (LR * LC + 255) / 256;
+    constexpr int NRUN = (LR * 4 + 255) / 256;
     __shared__ __attribute__((aligned(16))) float L[LR * LS];
-    __shared__ __attribute__((aligned(16))) float RM[LR * TILE_W];
     __shared__ uint64_t s_mask[TILE_H];
     __shared__ uint32_t s_off[TILE_H];
     __shared__ uint32_t s_lcnt;
-    if (threadIdx.x == 0) s_lcnt = 0;
 
+    const int tid = threadIdx.x;
+    const int a = dp.a;
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
+    const uint32_t f = b.tile_frame[item];
     const Frame fr = b.frames[f];
     const uint32_t local = item - fr.tile_base;
     const int H = (int)fr.H, W = (int)fr.W;
     const int x0 = (int)(local % fr.tx) * TILE_W;
     const int y0 = (int)(local / fr.tx) * TILE_H;
-    const int tid = threadIdx.x;
-    const int a = dp.a;
-    const float* __restrict__ P = fr.P;
+    const GLOBAL float* P = (const GLOBAL float*)fr.P;
+    if (tid == 0) s_lcnt = 0;
 
-    // stage P over the tile plus the window halo (-inf outside the frame);
-    // every load is issued before the first LDS store
-    constexpr int NFILL = (LR * LC + 255) / 256;
-    float fill[NFILL];
+    {
+        float fill[NFILL];
 #pragma unroll
-    for (int q = 0; q < NFILL; ++q) {
-        const int idx = tid + q * 256;
-        const int r = idx / LC, j = idx - r * LC;
-        const int u = y0 - a + r, v = x0 - a + j;
-        fill[q] = __uint_as_float(NEG_INF_BITS);
-        if (idx < LR * LC && u >= 0 && u < H && v >= 0 && v < W) fill[q] = P[(size_t)u * W + v];
-    }
+        for (int q = 0; q < NFILL; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / LC, j = idx - r * LC;
+            const int u = y0 - a + r, v = x0 - a + j;
+            fill[q] = __uint_as_float(NEG_INF_BITS);
+            if (idx < LR * LC && u >= 0 && u < H && v >= 0 && v < W) fill[q] = P[(size_t)u * W + v];
+        }
 #pragma unroll
-    for (int q = 0; q < NFILL; ++q) {
-        const int idx = tid + q * 256;
-        const int r = idx / LC, j = idx - r * LC;
-        if (idx < LR * LC) L[r * LS + j] = fill[q];
+        for (int q = 0; q < NFILL; ++q) {
+            const int idx = tid + q * 256;
+            const int r = idx / LC, j = idx - r * LC;
+            if (idx < LR * LC) L[r * LS + j] = fill[q];
+        }
     }
     __syncthreads();
 
-    // row pass: RM[r][c] = max(L[r][c .. c+KD-1])
-    for (int run = tid; run < LR * 4; run += 256) {
-        const int r = run >> 2, c0 = (run & 3) * N;
-        float x[NIN];
-        const float* src = L + r * LS + c0;
+    // centre values of this thread's 16 outputs, read before the overwrite
+    const int c = tid & 63, kw = tid >> 6;
+    float center[N];
 #pragma unroll
-        for (int m = 0; m < NV4; ++m) {
-            const float4 q = *(const float4*)(src + 4 * m);
-            if (4 * m + 0 < NIN) x[4 * m + 0] = q.x;
-            if (4 * m + 1 < NIN) x[4 * m + 1] = q.y;
-            if (4 * m + 2 < NIN) x[4 * m + 2] = q.z;
-            if (4 * m + 3 < NIN) x[4 * m + 3] = q.w;
+    for (int m = 0; m < N; ++m) center[m] = L[(kw * N + m + a) * LS + (c + a)];
+
+    // row pass: RM[r][c] = max(L[r][c .. c+KD-1]), written over L[r][0..63]
+    float y[NRUN][N];
+#pragma unroll
+    for (int k2 = 0; k2 < NRUN; ++k2) {
+        const int run = tid + k2 * 256;
+        if (run < LR * 4) {
+            const int r = run >> 2, c0 = (run & 3) * N;
+            float x[NIN];
+            const float* src = L + r * LS + c0;
+#pragma unroll
+            for (int m = 0; m < NV4; ++m) {
+                const float4 q = *(const float4*)(src + 4 * m);
+                if (4 * m + 0 < NIN) x[4 * m + 0] = q.x;
+                if (4 * m + 1 < NIN) x[4 * m + 1] = q.y;
+                if (4 * m + 2 < NIN) x[4 * m + 2] = q.z;
+                if (4 * m + 3 < NIN) x[4 * m + 3] = q.w;
+            }
+            sliding_max<N, KD>(x, y[k2]);
         }
-        float y[N];
-        sliding_max<N, KD>(x, y);
-        float* dst = RM + r * TILE_W + c0;
+    }
+    __syncthreads();
 #pragma unroll
-        for (int m = 0; m < N / 4; ++m)
-            *(float4*)(dst + 4 * m) = make_float4(y[4 * m], y[4 * m + 1], y[4 * m + 2], y[4 * m + 3]);
+    for (int k2 = 0; k2 < NRUN; ++k2) {
+        const int run = tid + k2 * 256;
+        if (run < LR * 4) {
+            const int r = run >> 2, c0 = (run & 3) * N;
+            float* dst = L + r * LS + c0;
+#pragma unroll
+            for (int m = 0; m < N / 4; ++m)
+                *(float4*)(dst + 4 * m) =
+                    make_float4(y[k2][4 * m], y[k2][4 * m + 1], y[k2][4 * m + 2], y[k2][4 * m + 3]);
+        }
     }
     __syncthreads();
 
     // column pass + candidate test: lane = column, wave = 16-row run
-    const int c = tid & 63, k = tid >> 6;
     const int v = x0 + c;
-    float x[NIN];
-#pragma unroll
-    for (int m = 0; m < NIN; ++m) x[m] = RM[(k * N + m) * TILE_W + c];
     float cm[N];
-    sliding_max<N, KD>(x, cm);
+    {
+        float x[NIN];
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) x[m] = L[(kw * N + m) * LS + c];
+        sliding_max<N, KD>(x, cm);
+    }
 
     int32_t bin_lo = -1, bin_hi = -1;
     if (dp.has_pct) {
         bin_lo = b.selstate[f * 4 + 0];
         bin_hi = b.selstate[f * 4 + 2];
     }
-    uint32_t* list = b.list + (size_t)item * TILE_PX;
+    GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
     uint32_t cbits = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        const int r = k * N + m;
+        const int r = kw * N + m;
         const int u = y0 + r;
-        const float center = L[(r + a) * LS + (c + a)];
         const bool inb = (u < H) && (v < W);
         const bool interior = (u - dp.band >= 0) && (u + dp.band < H) &&
                               (v - dp.band >= 0) && (v + dp.band < W);
-        const bool cand = inb && interior && (center >= cm[m]) &&
-                          !(dp.diw && center == 0.f);
+        const bool cand = inb && interior && (center[m] >= cm[m]) &&
+                          !(dp.diw && center[m] == 0.f);
         const uint64_t msk = __ballot(cand);
         if (c == 0) s_mask[r] = msk;
         cbits |= (cand ? 1u : 0u) << m;
         if (dp.has_pct) {
-            const uint32_t bits = __float_as_uint(center);
+            const uint32_t bits = __float_as_uint(center[m]);
             const int key = (int)pct_key(bits);
             const bool match = inb && key != 0 && (key == bin_lo || key == bin_hi);
             const uint64_t mm = __ballot(match);
@@ -728,11 +743,11 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         }
     }
     __syncthreads();
-    uint16_t* slots = b.slots + (size_t)item * TILE_PX;
+    GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
         if (cbits & (1u << m)) {
-            const int r = k * N + m;
+            const int r = kw * N + m;
             const uint32_t rank = s_off[r] + (uint32_t)__popcll(s_mask[r] & lanemask_lt(c));
             slots[rank] = (uint16_t)(r * TILE_W + c);
         }
@@ -752,7 +767,7 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
     float* L = dyn;                  // LR * LC
     float* RM = dyn + LR * LC;       // LR * TILE_W
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-    const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
+    const uint32_t f = b.tile_frame[item];
     const Frame fr = b.frames[f];
     const uint32_t local = item - fr.tile_base;
     const int H = (int)fr.H, W = (int)fr.W;
@@ -1009,7 +1024,7 @@ __global__ __launch_bounds__(256) void k_refine(Batch b, const Geom* __restrict_
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
     if (item >= b.tiles) return;
-    const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
+    const uint32_t f = b.tile_frame[item];
     const Frame fr = b.frames[f];
     const uint32_t local = item - fr.tile_base;
     const int H = (int)fr.H, W = (int)fr.W;
@@ -1187,7 +1202,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     Frame fr;
     int tx = 0, ty = 0, W = 0, y_lo = 0, y_hi = -1, x_lo = 0, x_hi = -1;
     if (active) {
-        f = find_frame(b.tile_bases, b.nframes, item);
+        f = b.tile_frame[item];
         fr = b.frames[f];
         const uint32_t local = item - fr.tile_base;
         W = (int)fr.W;
@@ -1394,7 +1409,7 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
     if (item >= b.tiles) return;
-    const uint32_t f = find_frame(b.tile_bases, b.nframes, item);
+    const uint32_t f = b.tile_frame[item];
     const Frame fr = b.frames[f];
     const uint32_t local = item - fr.tile_base;
     const int W = (int)fr.W;
