@@ -73,6 +73,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-percentile", action="store_true",
+                    help="locate without the percentile test (the reference's default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -85,6 +87,8 @@ def main() -> int:
     from tracking_optimizations_amd.gpu_tracking import make_params
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    if args.no_percentile:
+        del kw["percentile"]
     params = make_params(**kw)
     ctx = _lib.Context(0)
     if args.batch:
@@ -189,7 +193,8 @@ def main() -> int:
         sample = movie.to_host(np.uint16, (ns, H, W))
         cores = len(os.sched_getaffinity(0))
         threads = max(1, min(cores, 16))
-        okw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64.0, minmass=0.0)
+        okw = dict(diameter=11, separation=11, smoothing_size=11, minmass=0.0,
+                   percentile=None if args.no_percentile else 64.0)
         cfps, crows, cdt = cpu_baseline(sample, okw, threads)
         log(f"cpu oracle: {cfps:.2f} frames/s over {ns} frames on {threads} threads ({cdt:.1f} s)")
         cpu = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",

@@ -150,3 +150,15 @@ def test_empty_movie(ctx):
 def test_blank_frames(ctx):
     rows, cand, _, _ = locate(np.full((2, 128, 128), 7, np.uint16), make_params(9), ctx=ctx)
     assert len(rows) == 0
+
+
+@pytest.mark.parametrize("per_launch", [1, 2])
+def test_multi_batch_two_streams_identical(frames512, per_launch):
+    """Small launches force several batches on alternating streams; rows (and
+    their order) must not change."""
+    p = make_params(**CASES["c2"])
+    ref = locate(frames512, p, ctx=_lib.Context(0))
+    ctx = _lib.Context(0)
+    ctx.set_batch(per_launch)
+    got = locate(frames512, p, ctx=ctx)
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])

@@ -247,23 +247,38 @@ const char* const kColFrame = "frame";
 
 // ---- context --------------------------------------------------------------
 
+// Device buffers of one in-flight batch.  Consecutive batches alternate
+// between two workspaces and two streams so that one batch's
+// compute-bound bandpass overlaps the next batch's latency-bound stages.
+struct Workspace {
+    DevBuf in_stage, P, tmp, list, slots, rec, keep, tile3, perframe;
+    void release() {
+        DevBuf* b[] = {&in_stage, &P, &tmp, &list, &slots, &rec, &keep, &tile3, &perframe};
+        for (DevBuf* x : b) x->release();
+    }
+};
+
+constexpr int kStreams = 2;
+
 struct gt_context {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;            // = streams[0]
+    hipStream_t streams[kStreams] = {nullptr, nullptr};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_join = nullptr;
+    hipEvent_t ev_scan[kStreams] = {nullptr, nullptr};
     gt_timing last{};
     uint32_t batch_frames = 0;   // 0 = automatic
-    size_t workspace_limit = (size_t)6 << 30;
+    size_t workspace_limit = (size_t)6 << 30;   // per workspace
     int seg_rows = 64;
-    // per-stage event timing
+    // per-stage event timing (forces one stream so that kernels do not overlap)
     int profiling = 0;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     double stage_ms[GT_NSTAGES] = {0};
     uint64_t stage_launches[GT_NSTAGES] = {0};
-    // workspaces
-    DevBuf geom, frames_tab, bases, in_stage, P, tmp, list, slots, rec, keep;
-    DevBuf tile3, perframe, rows, rows_cand, counters;
+    DevBuf geom, frames_tab, bases, rows, rows_cand, counters;
+    Workspace ws[kStreams];
 };
 
 struct gt_result {
@@ -345,15 +360,15 @@ hipEvent_t take_event(gt_context* ctx) {
 
 // Runs one stage launch; with profiling on, brackets it with events.
 template <class F>
-hipError_t stage(gt_context* ctx, int id, F&& launch) {
+hipError_t stage(gt_context* ctx, hipStream_t st, int id, F&& launch) {
     if (!ctx->profiling) return launch();
     hipEvent_t a = take_event(ctx), b = take_event(ctx);
     if (!a || !b) return hipErrorOutOfMemory;
-    hipError_t e = hipEventRecord(a, ctx->stream);
+    hipError_t e = hipEventRecord(a, st);
     if (e != hipSuccess) return e;
     e = launch();
     if (e != hipSuccess) return e;
-    e = hipEventRecord(b, ctx->stream);
+    e = hipEventRecord(b, st);
     ctx->ev_used.push_back({id, {a, b}});
     return e;
 }
@@ -463,23 +478,27 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     ctx->last.pixels = total_px;
 
     // ---- workspaces -------------------------------------------------------
+    const int nstreams = (plain || ctx->profiling || plan.size() < 2) ? 1 : kStreams;
     const uint64_t slot_cap = maxneed.tiles * TILE_PX;
     const size_t nfb = maxneed.nframes;
-    HIP_TRY(ctx->geom.ensure(sizeof(Geom)));
-    HIP_TRY(ctx->P.ensure(maxneed.px * 4));
-    if (generic) HIP_TRY(ctx->tmp.ensure(maxneed.px * 8));
-    if (g.has_pct) HIP_TRY(ctx->list.ensure(slot_cap * 4));
-    HIP_TRY(ctx->slots.ensure(slot_cap * 2));
-    HIP_TRY(ctx->rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
-    HIP_TRY(ctx->keep.ensure(slot_cap));
-    if (!on_device) HIP_TRY(ctx->in_stage.ensure(maxneed.in_bytes));
-    HIP_TRY(ctx->tile3.ensure(al256(maxneed.tiles * 4) * 4));
     // per-frame: hist (4096 u32), frame_rows, frame_cands, spare, selstate(4), thr(double), row_base(u64)
     const size_t pf_hist = al256(nfb * HIST_BINS * 4);
     const size_t pf_small = al256(nfb * 4);
     const size_t pf_sel = al256(nfb * 16);
     const size_t pf_8 = al256(nfb * 8);
-    HIP_TRY(ctx->perframe.ensure(pf_hist + 3 * pf_small + pf_sel + 2 * pf_8));
+    HIP_TRY(ctx->geom.ensure(sizeof(Geom)));
+    for (int si = 0; si < nstreams; ++si) {
+        Workspace& w = ctx->ws[si];
+        HIP_TRY(w.P.ensure(maxneed.px * 4));
+        if (generic) HIP_TRY(w.tmp.ensure(maxneed.px * 8));
+        if (g.has_pct) HIP_TRY(w.list.ensure(slot_cap * 4));
+        HIP_TRY(w.slots.ensure(slot_cap * 2));
+        HIP_TRY(w.rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
+        HIP_TRY(w.keep.ensure(slot_cap));
+        if (!on_device) HIP_TRY(w.in_stage.ensure(maxneed.in_bytes));
+        HIP_TRY(w.tile3.ensure(al256(maxneed.tiles * 4) * 4));
+        HIP_TRY(w.perframe.ensure(pf_hist + 3 * pf_small + pf_sel + 2 * pf_8));
+    }
     HIP_TRY(ctx->counters.ensure(256));
 
     // rows capacity: one row per 16 pixels, grown on overflow
@@ -494,6 +513,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     std::vector<size_t> bases_off(plan.size());
     for (size_t bi = 0; bi < plan.size(); ++bi) {
         const auto& bp = plan[bi];
+        Workspace& wsb = ctx->ws[bi % nstreams];
         uint64_t pxo = 0, inoff = 0;
         uint32_t k1b = 0, tb = 0;
         std::vector<uint32_t> k1v, tbv;
@@ -504,11 +524,11 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             if (on_device) {
                 F.in = frames[fi].ptr;
             } else {
-                F.in = (const char*)ctx->in_stage.p + inoff;
+                F.in = (const char*)wsb.in_stage.p + inoff;
                 inoff += al256(fs.px * es);
             }
-            F.P = ctx->P.as<float>() + pxo;
-            F.tmp = generic ? ctx->tmp.as<float>() + 2 * pxo : nullptr;
+            F.P = wsb.P.as<float>() + pxo;
+            F.tmp = generic ? wsb.tmp.as<float>() + 2 * pxo : nullptr;
             F.H = frames[fi].h;
             F.W = frames[fi].w;
             F.k1_tx = fs.k1_tx;
@@ -552,12 +572,20 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         res->cand.clear();
         launches = 0;
 
-        HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(launch_fill_u64(counters, 2, 0, ctx->stream));
+        hipStream_t s0 = ctx->streams[0];
+        HIP_TRY(hipEventRecord(ctx->ev0, s0));
+        HIP_TRY(launch_fill_u64(counters, 2, 0, s0));
         ++launches;
+        if (nstreams > 1) {
+            HIP_TRY(hipEventRecord(ctx->ev_join, s0));
+            HIP_TRY(hipStreamWaitEvent(ctx->streams[1], ctx->ev_join, 0));
+        }
         uint64_t host_rows_done = 0;
         for (size_t bi = 0; bi < plan.size(); ++bi) {
             const auto& bp = plan[bi];
+            const int si = (int)(bi % nstreams);
+            hipStream_t st = ctx->streams[si];
+            Workspace& w = ctx->ws[si];
             const uint32_t nb = (uint32_t)(bp.b1 - bp.b0);
             Batch b;
             std::memset(&b, 0, sizeof(b));
@@ -571,25 +599,25 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             b.tile_frame = b.k1_frame + b.k1_items;
             b.in_type = in_type;
             b.vec = vec ? 1 : 0;
-            char* t3 = ctx->tile3.as<char>();
+            char* t3 = w.tile3.as<char>();
             const size_t t3s = al256(maxneed.tiles * 4);
             b.tile_count = (uint32_t*)t3;
             b.tile_keep = (uint32_t*)(t3 + t3s);
             b.tile_row_off = (uint32_t*)(t3 + 2 * t3s);
             b.tile_lcount = (uint32_t*)(t3 + 3 * t3s);
-            char* pf = ctx->perframe.as<char>();
+            char* pf = w.perframe.as<char>();
             b.hist1 = g.has_pct ? (uint32_t*)pf : nullptr;
             pf += pf_hist;
-            b.list = g.has_pct ? ctx->list.as<uint32_t>() : nullptr;
+            b.list = g.has_pct ? w.list.as<uint32_t>() : nullptr;
             b.frame_rows = (uint32_t*)pf; pf += pf_small;
             b.frame_cands = (uint32_t*)pf; pf += pf_small;
             pf += pf_small;
             b.selstate = (int32_t*)pf; pf += pf_sel;
             b.thr = (double*)pf; pf += pf_8;
             b.frame_row_base = (uint64_t*)pf;
-            b.slots = ctx->slots.as<uint16_t>();
-            b.rec = ctx->rec.as<float>();
-            b.keep = ctx->keep.as<uint8_t>();
+            b.slots = w.slots.as<uint16_t>();
+            b.rec = w.rec.as<float>();
+            b.keep = w.keep.as<uint8_t>();
             b.slot_cap = slot_cap;
             b.cand_total = counters;
             b.row_carry = counters + 1;
@@ -601,51 +629,55 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                 size_t off = 0;
                 for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
                     const size_t nbytes = (size_t)frames[fi].h * frames[fi].w * es;
-                    HIP_TRY(hipMemcpyAsync((char*)ctx->in_stage.p + off, frames[fi].ptr, nbytes,
-                                           hipMemcpyHostToDevice, ctx->stream));
+                    HIP_TRY(hipMemcpyAsync((char*)w.in_stage.p + off, frames[fi].ptr, nbytes,
+                                           hipMemcpyHostToDevice, st));
                     off += al256(nbytes);
                 }
             }
-            HIP_TRY(stage(ctx, GT_STAGE_ZERO, [&] { return launch_zero_batch(b, ctx->stream); }));
+            HIP_TRY(stage(ctx, st, GT_STAGE_ZERO, [&] { return launch_zero_batch(b, st); }));
             ++launches;
             if (generic) {
                 uint32_t maxpx = 0;
                 for (size_t fi = bp.b0; fi < bp.b1; ++fi)
                     maxpx = std::max<uint32_t>(maxpx, frames[fi].h * frames[fi].w);
-                HIP_TRY(stage(ctx, GT_STAGE_BANDPASS, [&] {
-                    return launch_bandpass_generic(b, d_geom, maxpx, g.has_pct, ctx->stream);
+                HIP_TRY(stage(ctx, st, GT_STAGE_BANDPASS, [&] {
+                    return launch_bandpass_generic(b, d_geom, maxpx, g.has_pct, st);
                 }));
                 launches += 2;
             } else {
-                HIP_TRY(stage(ctx, GT_STAGE_BANDPASS,
-                              [&] { return launch_bandpass(b, g, d_geom, seg, ctx->stream); }));
+                HIP_TRY(stage(ctx, st, GT_STAGE_BANDPASS,
+                              [&] { return launch_bandpass(b, g, d_geom, seg, st); }));
                 ++launches;
             }
             if (g.has_pct) {
-                HIP_TRY(stage(ctx, GT_STAGE_SELECT1, [&] { return launch_select1(b, g, ctx->stream); }));
+                HIP_TRY(stage(ctx, st, GT_STAGE_SELECT1, [&] { return launch_select1(b, g, st); }));
                 ++launches;
             }
-            HIP_TRY(stage(ctx, GT_STAGE_DETECT, [&] { return launch_detect(b, g, d_geom, ctx->stream); }));
+            HIP_TRY(stage(ctx, st, GT_STAGE_DETECT, [&] { return launch_detect(b, g, d_geom, st); }));
             ++launches;
             if (g.has_pct) {
-                HIP_TRY(stage(ctx, GT_STAGE_SELECT2, [&] { return launch_select2(b, g, ctx->stream); }));
+                HIP_TRY(stage(ctx, st, GT_STAGE_SELECT2, [&] { return launch_select2(b, g, st); }));
                 ++launches;
             }
-            HIP_TRY(stage(ctx, GT_STAGE_REFINE, [&] { return launch_refine(b, g, d_geom, ctx->stream); }));
-            HIP_TRY(stage(ctx, GT_STAGE_DEDUP, [&] { return launch_dedup(b, g, d_geom, ctx->stream); }));
-            HIP_TRY(stage(ctx, GT_STAGE_SCAN, [&] { return launch_tile_scan(b, ctx->stream); }));
-            HIP_TRY(stage(ctx, GT_STAGE_SCAN, [&] { return launch_frame_scan(b, ctx->stream); }));
-            HIP_TRY(stage(ctx, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, ctx->stream); }));
+            HIP_TRY(stage(ctx, st, GT_STAGE_REFINE, [&] { return launch_refine(b, g, d_geom, st); }));
+            HIP_TRY(stage(ctx, st, GT_STAGE_DEDUP, [&] { return launch_dedup(b, g, d_geom, st); }));
+            HIP_TRY(stage(ctx, st, GT_STAGE_SCAN, [&] { return launch_tile_scan(b, st); }));
+            // row bases continue the previous batch's count: order the scans
+            if (nstreams > 1 && bi > 0)
+                HIP_TRY(hipStreamWaitEvent(st, ctx->ev_scan[(bi - 1) % nstreams], 0));
+            HIP_TRY(stage(ctx, st, GT_STAGE_SCAN, [&] { return launch_frame_scan(b, st); }));
+            if (nstreams > 1) HIP_TRY(hipEventRecord(ctx->ev_scan[si], st));
+            HIP_TRY(stage(ctx, st, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, st); }));
             launches += 5;
 
             if (plain && result_on_device) {
                 // per-frame completion, as the reference's get_work waits per frame
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                HIP_TRY(hipStreamSynchronize(st));
             } else if (plain) {
                 // per-frame readback, as the reference's get_work does
                 uint64_t carry = 0;
-                HIP_TRY(hipMemcpyAsync(&carry, counters + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                HIP_TRY(hipMemcpyAsync(&carry, counters + 1, 8, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
                 const uint64_t upto = std::min(carry, rows_cap);
                 if (upto > host_rows_done) {
                     const uint64_t n = upto - host_rows_done;
@@ -653,19 +685,23 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                     res->cand.resize(upto);
                     HIP_TRY(hipMemcpyAsync(res->data.data() + host_rows_done * g.ncols,
                                            ctx->rows.as<float>() + host_rows_done * g.ncols,
-                                           n * g.ncols * 4, hipMemcpyDeviceToHost, ctx->stream));
+                                           n * g.ncols * 4, hipMemcpyDeviceToHost, st));
                     HIP_TRY(hipMemcpyAsync(res->cand.data() + host_rows_done,
                                            ctx->rows_cand.as<int64_t>() + host_rows_done, n * 8,
-                                           hipMemcpyDeviceToHost, ctx->stream));
-                    HIP_TRY(hipStreamSynchronize(ctx->stream));
+                                           hipMemcpyDeviceToHost, st));
+                    HIP_TRY(hipStreamSynchronize(st));
                     host_rows_done = upto;
                 }
             }
         }
-        HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+        if (nstreams > 1) {
+            HIP_TRY(hipEventRecord(ctx->ev_join, ctx->streams[1]));
+            HIP_TRY(hipStreamWaitEvent(s0, ctx->ev_join, 0));
+        }
+        HIP_TRY(hipEventRecord(ctx->ev1, s0));
         uint64_t cnt[2] = {0, 0};
-        HIP_TRY(hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, s0));
+        HIP_TRY(hipStreamSynchronize(s0));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last.device_ms = ms;
@@ -812,11 +848,17 @@ gt_status gt_context_create(int device, gt_context** out) {
     HIP_TRY(hipSetDevice(device));
     gt_context* c = new gt_context();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    bool ok = true;
+    for (int i = 0; i < kStreams; ++i)
+        ok = ok && hipStreamCreateWithFlags(&c->streams[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->ev_scan[i], hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
+         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
         delete c;
         return fail(GT_E_HIP, "stream/event creation failed");
     }
+    c->stream = c->streams[0];
     *out = c;
     return GT_OK;
 }
@@ -824,16 +866,20 @@ gt_status gt_context_create(int device, gt_context** out) {
 gt_status gt_context_destroy(gt_context* ctx) {
     if (!ctx) return GT_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->in_stage, &ctx->P,
-                      &ctx->tmp, &ctx->list, &ctx->slots, &ctx->rec, &ctx->keep, &ctx->tile3,
-                      &ctx->perframe, &ctx->rows, &ctx->rows_cand, &ctx->counters};
+    for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(ctx->streams[i]);
+    DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->rows, &ctx->rows_cand,
+                      &ctx->counters};
     for (DevBuf* b : bufs) b->release();
+    for (int i = 0; i < kStreams; ++i) ctx->ws[i].release();
     collect_stage_times(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
-    (void)hipStreamDestroy(ctx->stream);
+    (void)hipEventDestroy(ctx->ev_join);
+    for (int i = 0; i < kStreams; ++i) {
+        (void)hipEventDestroy(ctx->ev_scan[i]);
+        (void)hipStreamDestroy(ctx->streams[i]);
+    }
     delete ctx;
     return GT_OK;
 }
@@ -1024,7 +1070,7 @@ gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t by
 
 gt_status gt_synchronize(gt_context* ctx) {
     if (!ctx) return fail(GT_E_INVALID, "NULL argument");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < kStreams; ++i) HIP_TRY(hipStreamSynchronize(ctx->streams[i]));
     return GT_OK;
 }
 

@@ -888,68 +888,77 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 // ---------------------------------------------------------------------------
 // select2: exact order statistics from the value list, then the threshold
 // ---------------------------------------------------------------------------
-// Radix selection of the `rank`-th smallest value of histogram bin `bin`
-// among the per-tile value lists of one frame (bits [18:8], then [7:0]).
-// Each wave walks whole tiles; lanes stride over the tile's list.
-__device__ uint32_t select_in_list(const Batch& b, const Frame& fr, int bin, int rank,
-                                   uint32_t* s_hist /* >= 2048 */, uint32_t* s_res) {
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int lane = t & 63, wave = t >> 6, nw = nt >> 6;
-    const uint32_t ntiles = fr.tx * fr.ty;
-    for (int pass = 0; pass < 2; ++pass) {
-        const int nb = pass == 0 ? 2048 : 256;
-        for (int i = t; i < nb; i += nt) s_hist[i] = 0;
-        __syncthreads();
-        const uint32_t sa = pass == 0 ? 0u : s_res[0];
-        for (uint32_t tl = wave; tl < ntiles; tl += nw) {
-            const uint32_t it = fr.tile_base + tl;
-            const uint32_t cnt = b.tile_lcount[it];
-            const uint32_t* list = b.list + (size_t)it * TILE_PX;
-            for (uint32_t i = lane; i < cnt; i += 64) {
-                const uint32_t bits = list[i];
-                if ((int)pct_key(bits) != bin) continue;
-                if (pass == 0) atomicAdd(&s_hist[(bits >> 8) & 2047u], 1u);
-                else if (((bits >> 8) & 2047u) == sa) atomicAdd(&s_hist[bits & 255u], 1u);
-            }
-        }
-        __syncthreads();
-        if (t == 0) {
-            const uint32_t want = pass == 0 ? (uint32_t)rank : s_res[1];
-            uint32_t cum = 0, sel = 0;
-            for (int i = 0; i < nb; ++i) {
-                if (want < cum + s_hist[i]) { sel = (uint32_t)i; break; }
-                cum += s_hist[i];
-            }
-            if (pass == 0) {
-                s_res[0] = sel;
-                s_res[1] = want - cum;
-            } else {
-                s_res[2] = ((uint32_t)(bin - 1) << 19) | (s_res[0] << 8) | sel;
-            }
-        }
-        __syncthreads();
-    }
-    const uint32_t v = s_res[2];
-    __syncthreads();
-    return v;
-}
-
+// Exact order statistics lo and hi of one frame by radix selection over the
+// per-tile value lists: level A counts bits [18:8] of the values in each
+// target bin, level B bits [7:0] of the values sharing the chosen level-A
+// digit.  Both targets share every pass; each thread walks whole tile lists
+// with 16-byte loads.
 __global__ __launch_bounds__(1024) void k_select2(Batch b, float percentile) {
-    __shared__ uint32_t s_hist[2048];
-    __shared__ uint32_t s_res[4];
+    __shared__ uint32_t s_ha[2][2048];
+    __shared__ uint32_t s_res[2][3];
     const uint32_t f = blockIdx.x;
     const Frame fr = b.frames[f];
     const uint64_t n = (uint64_t)fr.H * fr.W;
     uint64_t lo, hi;
     double frac;
     pct_ranks(n, percentile, lo, hi, frac);
-    const int bin_lo = b.selstate[f * 4 + 0], r_lo = b.selstate[f * 4 + 1];
-    const int bin_hi = b.selstate[f * 4 + 2], r_hi = b.selstate[f * 4 + 3];
-    float vlo = 0.f, vhi = 0.f;
-    if (bin_lo != 0) vlo = __uint_as_float(select_in_list(b, fr, bin_lo, r_lo, s_hist, s_res));
-    if (hi == lo) vhi = vlo;
-    else if (bin_hi != 0) vhi = __uint_as_float(select_in_list(b, fr, bin_hi, r_hi, s_hist, s_res));
-    if (threadIdx.x == 0) b.thr[f] = (double)vlo + frac * ((double)vhi - (double)vlo);
+    const int t = threadIdx.x;
+    const int bin[2] = {b.selstate[f * 4 + 0], b.selstate[f * 4 + 2]};
+    const int rank[2] = {b.selstate[f * 4 + 1], b.selstate[f * 4 + 3]};
+    const bool need[2] = {bin[0] != 0, hi != lo && bin[1] != 0};
+    const uint32_t ntiles = fr.tx * fr.ty;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int nb = pass == 0 ? 2048 : 256;
+        for (int i = t; i < 2 * 2048; i += 1024) s_ha[i / 2048][i % 2048] = 0;
+        __syncthreads();
+        const uint32_t sa0 = pass ? s_res[0][0] : 0u, sa1 = pass ? s_res[1][0] : 0u;
+        for (uint32_t tl = t; tl < ntiles; tl += 1024) {
+            const uint32_t it = fr.tile_base + tl;
+            const uint32_t cnt = b.tile_lcount[it];
+            const GLOBAL uint4* lst = (const GLOBAL uint4*)(b.list + (size_t)it * TILE_PX);
+            for (uint32_t i4 = 0; i4 < (cnt + 3) / 4; ++i4) {
+                const uint4 q = lst[i4];
+                const uint32_t vals[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    if (i4 * 4 + e >= cnt) break;
+                    const uint32_t bits = vals[e];
+                    const int key = (int)pct_key(bits);
+                    const uint32_t da = (bits >> 8) & 2047u;
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        if (!need[s] || key != bin[s]) continue;
+                        if (pass == 0) atomicAdd(&s_ha[s][da], 1u);
+                        else if (da == (s ? sa1 : sa0)) atomicAdd(&s_ha[s][bits & 255u], 1u);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        const bool need_t = t == 0 ? need[0] : need[1];
+        if (t < 2 && need_t) {
+            const int bin_t = t == 0 ? bin[0] : bin[1];
+            const int rank_t = t == 0 ? rank[0] : rank[1];
+            const uint32_t want = pass == 0 ? (uint32_t)rank_t : s_res[t][1];
+            uint32_t cum = 0, sel = 0;
+            for (int i = 0; i < nb; ++i) {
+                if (want < cum + s_ha[t][i]) { sel = (uint32_t)i; break; }
+                cum += s_ha[t][i];
+            }
+            if (pass == 0) {
+                s_res[t][0] = sel;
+                s_res[t][1] = want - cum;
+            } else {
+                s_res[t][2] = ((uint32_t)(bin_t - 1) << 19) | (s_res[t][0] << 8) | sel;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const float vlo = need[0] ? __uint_as_float(s_res[0][2]) : 0.f;
+        const float vhi = (hi == lo) ? vlo : (need[1] ? __uint_as_float(s_res[1][2]) : 0.f);
+        b.thr[f] = (double)vlo + frac * ((double)vhi - (double)vlo);
+    }
 }
 
 hipError_t launch_select2(const Batch& b, const Geom& g, hipStream_t s) {
@@ -1019,29 +1028,73 @@ __device__ __forceinline__ void disk_any(const float* P, int H, int W, int band,
     else disk_eval(P, H, W, band, rint, u, v, m, c0, c1);
 }
 
+// Candidates of RT_TILES consecutive tiles are packed over the 256 lanes of a
+// workgroup (a tile holds ~16 candidates on average; one wave per tile would
+// leave three quarters of the lanes idle while issuing the same number of
+// scattered loads).
+constexpr int RT_TILES = 16;
+
+struct RefTile {
+    const float* P;
+    const void* in;
+    int x0, y0, H, W;
+    uint32_t f, cnt, pref;
+};
+
 template <int RI>
 __global__ __launch_bounds__(256) void k_refine(Batch b, const Geom* __restrict__ g) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t item = blockIdx.x * 4 + wave;
-    if (item >= b.tiles) return;
-    const uint32_t f = b.tile_frame[item];
-    const Frame fr = b.frames[f];
-    const uint32_t local = item - fr.tile_base;
-    const int H = (int)fr.H, W = (int)fr.W;
-    const int x0 = (int)(local % fr.tx) * TILE_W;
-    const int y0 = (int)(local / fr.tx) * TILE_H;
-    const uint32_t cnt = b.tile_count[item];
-    const size_t slot0 = (size_t)item * TILE_PX;
-    const float* __restrict__ P = fr.P;
+    __shared__ RefTile s_t[RT_TILES];
+    __shared__ uint32_t s_total;
+    const uint32_t item0 = blockIdx.x * RT_TILES;
+    const int nt = (int)min<uint32_t>(RT_TILES, b.tiles - item0);
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        uint32_t cnt = 0;
+        RefTile rt;
+        if (tid < nt) {
+            const uint32_t item = item0 + tid;
+            const uint32_t f = b.tile_frame[item];
+            const Frame fr = b.frames[f];
+            const uint32_t local = item - fr.tile_base;
+            rt.P = fr.P;
+            rt.in = fr.in;
+            rt.x0 = (int)(local % fr.tx) * TILE_W;
+            rt.y0 = (int)(local / fr.tx) * TILE_H;
+            rt.H = (int)fr.H;
+            rt.W = (int)fr.W;
+            rt.f = f;
+            cnt = b.tile_count[item];
+            rt.cnt = cnt;
+        }
+        const uint32_t inc = wave_incl_scan(cnt, tid);
+        if (tid < nt) {
+            rt.pref = inc - cnt;
+            s_t[tid] = rt;
+        }
+        if (tid == 63) s_total = inc;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
     const int band = g->band, rint = g->rint;
     const uint32_t maxit = g->max_iterations;
     const float minmass = g->minmass;
-    const double thr = g->has_pct ? b.thr[f] : 0.0;
     const float sthr = 0.6f;
     float* rec = b.rec;
     const uint64_t cap = b.slot_cap;
 
-    for (uint32_t i = lane; i < cnt; i += 64) {
+    for (uint32_t gi = tid; gi < total; gi += 256) {
+        int j = 0;
+#pragma unroll
+        for (int q = 1; q < RT_TILES; ++q)
+            if (q < nt && s_t[q].pref <= gi) j = q;
+        const RefTile& rt = s_t[j];
+        const uint32_t i = gi - rt.pref;
+        const uint32_t item = item0 + (uint32_t)j;
+        const size_t slot0 = (size_t)item * TILE_PX;
+        const int H = rt.H, W = rt.W;
+        const float* __restrict__ P = rt.P;
+        const double thr = g->has_pct ? b.thr[rt.f] : 0.0;
+        const int x0 = rt.x0, y0 = rt.y0;
         const uint32_t l = b.slots[slot0 + i];
         const int u0 = y0 + (int)(l / TILE_W), v0 = x0 + (int)(l % TILE_W);
         bool valid = true;
@@ -1139,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_refine(Batch b, const Geom* __restrict_
                         const float d = P[idx];
                         ms = ms + d;
                         rg = __builtin_fmaf(d, (float)(di * di + dj * dj), rg);
-                        rawm = rawm + raw_px(fr.in, b.in_type, idx);
+                        rawm = rawm + raw_px(rt.in, b.in_type, idx);
                         sg = (d > sg) ? d : sg;
                         es = __builtin_fmaf(d, g->sin2_gpu[tap], es);
                         ec = __builtin_fmaf(d, g->cos2_gpu[tap], ec);
@@ -1161,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_refine(Batch b, const Geom* __restrict_
 }
 
 hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
-    const dim3 grid((b.tiles + 3) / 4);
+    const dim3 grid((b.tiles + RT_TILES - 1) / RT_TILES);
     switch (g.rint) {
         case 2: hipLaunchKernelGGL(k_refine<2>, grid, dim3(256), 0, s, b, d_geom); break;
         case 3: hipLaunchKernelGGL(k_refine<3>, grid, dim3(256), 0, s, b, d_geom); break;
@@ -1175,12 +1228,14 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 }
 
 // ---------------------------------------------------------------------------
-// K4 dedup: one wave per tile.  The records of the tiles that can hold a
-// neighbour (start pixels within sep + 2*(max_iterations + rint) per axis) are
-// staged in LDS with parallel loads, then every own candidate scans them.
-// Neighbourhoods larger than the staging area fall back to reading the
-// records from global memory.  The last wave of a frame scans its tiles'
-// keep counts into row offsets.
+// K4 dedup: one wave per tile.  The valid records (mass != 0) of the tiles
+// that can hold a neighbour, and that lie within `separation` of the
+// bounding box of the tile's own valid positions, are staged in LDS, one per
+// lane.  Each own candidate is then broadcast to the wave and every lane
+// tests one staged record against it (execute_gpu.rs:258-284: drop when a
+// record strictly within `separation` has a larger mass, or an equal mass and
+// a lower raster index).  Staging areas that would overflow fall back to
+// scanning the records in global memory.
 // ---------------------------------------------------------------------------
 constexpr int DD_CAP = 512;   // staged records per wave
 
@@ -1189,128 +1244,159 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     __shared__ int32_t s_i[4][DD_CAP];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
-    const bool active = item < b.tiles;
+    if (item >= b.tiles) return;
     const uint64_t cap = b.slot_cap;
-    const float* rmass = b.rec + REC_MASS * cap;
-    const float* ry = b.rec + REC_Y * cap;
-    const float* rx = b.rec + REC_X * cap;
+    const GLOBAL float* rmass = (const GLOBAL float*)(b.rec + REC_MASS * cap);
+    const GLOBAL float* ry = (const GLOBAL float*)(b.rec + REC_Y * cap);
+    const GLOBAL float* rx = (const GLOBAL float*)(b.rec + REC_X * cap);
+    const GLOBAL uint16_t* slots = (const GLOBAL uint16_t*)b.slots;
     const float rad = g->sep;
     const float r2 = rad * rad;
     const int rty = g->dedup_rty, rtx = g->dedup_rtx;
 
-    uint32_t f = 0, cnt = 0;
-    Frame fr;
-    int tx = 0, ty = 0, W = 0, y_lo = 0, y_hi = -1, x_lo = 0, x_hi = -1;
-    if (active) {
-        f = b.tile_frame[item];
-        fr = b.frames[f];
-        const uint32_t local = item - fr.tile_base;
-        W = (int)fr.W;
-        tx = (int)(local % fr.tx);
-        ty = (int)(local / fr.tx);
-        cnt = b.tile_count[item];
-        y_lo = max(0, ty - rty);
-        y_hi = min((int)fr.ty - 1, ty + rty);
-        x_lo = max(0, tx - rtx);
-        x_hi = min((int)fr.tx - 1, tx + rtx);
-    }
+    const uint32_t f = b.tile_frame[item];
+    const Frame fr = b.frames[f];
+    const uint32_t local = item - fr.tile_base;
+    const int W = (int)fr.W;
+    const int tx = (int)(local % fr.tx), ty = (int)(local / fr.tx);
+    const uint32_t cnt = b.tile_count[item];
+    const size_t slot0 = (size_t)item * TILE_PX;
+    const int y_lo = max(0, ty - rty), y_hi = min((int)fr.ty - 1, ty + rty);
+    const int x_lo = max(0, tx - rtx), x_hi = min((int)fr.tx - 1, tx + rtx);
     const int nxr = x_hi - x_lo + 1;
-    const int ntl = active ? (y_hi - y_lo + 1) * nxr : 0;
-    bool staged = active && ntl <= 64;
-    uint32_t ncnt = 0, it2 = 0;
-    if (staged && lane < ntl) {
-        const int ny = y_lo + lane / nxr, nx = x_lo + lane % nxr;
-        it2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-        ncnt = b.tile_count[it2];
-    }
-    const uint32_t incl = wave_incl_scan(ncnt, lane);
-    const uint32_t excl = incl - ncnt;
-    const uint32_t M = __shfl(incl, 63, 64);
-    staged = staged && M <= (uint32_t)DD_CAP;
-    if (staged) {
-        // every lane stays in the loop so that the shuffles read live lanes
-        for (uint32_t rb = 0; rb < M; rb += 64) {
-            const uint32_t r = rb + lane;
-            int j = 0;
-            for (int q = 1; q < ntl; ++q)
-                if (__shfl(excl, q, 64) <= r) j = q;
-            const uint32_t t2 = __shfl(it2, j, 64);
-            const uint32_t e2 = __shfl(excl, j, 64);
-            if (r >= M) continue;
-            const size_t s = (size_t)t2 * TILE_PX + (r - e2);
-            const uint32_t l2 = b.slots[s];
-            const uint32_t loc2 = t2 - fr.tile_base;
-            const int py = (int)(loc2 / fr.tx) * TILE_H + (int)(l2 / TILE_W);
-            const int px = (int)(loc2 % fr.tx) * TILE_W + (int)(l2 % TILE_W);
-            s_m[wave][r] = rmass[s];
-            s_y[wave][r] = ry[s];
-            s_x[wave][r] = rx[s];
-            s_i[wave][r] = py * W + px;
+    const int ntl = (y_hi - y_lo + 1) * nxr;
+
+    // bounding box of the tile's own valid positions
+    float bymin = INFINITY, bymax = -INFINITY, bxmin = INFINITY, bxmax = -INFINITY;
+    for (uint32_t i = lane; i < cnt; i += 64) {
+        if (rmass[slot0 + i] != 0.f) {
+            const float y = ry[slot0 + i], x = rx[slot0 + i];
+            bymin = fminf(bymin, y); bymax = fmaxf(bymax, y);
+            bxmin = fminf(bxmin, x); bxmax = fmaxf(bxmax, x);
         }
     }
-    __syncthreads();
-
+    for (int o = 32; o > 0; o >>= 1) {
+        bymin = fminf(bymin, __shfl_xor(bymin, o, 64));
+        bymax = fmaxf(bymax, __shfl_xor(bymax, o, 64));
+        bxmin = fminf(bxmin, __shfl_xor(bxmin, o, 64));
+        bxmax = fmaxf(bxmax, __shfl_xor(bxmax, o, 64));
+    }
     uint32_t kept = 0;
-    if (active) {
-        const size_t slot0 = (size_t)item * TILE_PX;
-        for (uint32_t ib = 0; ib < cnt; ib += 64) {
-            const uint32_t i = ib + lane;
-            const bool own = i < cnt;
-            float m = 0.f, y = 0.f, x = 0.f;
-            int32_t idx = -1;
-            if (own) {
-                m = rmass[slot0 + i];
-                y = ry[slot0 + i];
-                x = rx[slot0 + i];
-                const uint32_t l = b.slots[slot0 + i];
-                idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
+    if (!(bymin <= bymax)) {  // no valid own record
+        for (uint32_t i = lane; i < cnt; i += 64) b.keep[slot0 + i] = 0;
+        if (lane == 0) b.tile_keep[item] = 0;
+        return;
+    }
+    const float margin = rad + 1.0f;
+    bymin -= margin; bymax += margin; bxmin -= margin; bxmax += margin;
+
+    // stage the neighbourhood (wave-uniform loops; lanes take records)
+    uint32_t S = 0;
+    bool staged = ntl <= 64;
+    if (staged) {
+        for (int q = 0; q < ntl && staged; ++q) {
+            const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
+            const uint32_t it2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+            const uint32_t c2 = b.tile_count[it2];
+            const size_t s2 = (size_t)it2 * TILE_PX;
+            for (uint32_t jb = 0; jb < c2; jb += 64) {
+                const uint32_t j = jb + lane;
+                bool take = false;
+                float m = 0.f, y = 0.f, x = 0.f;
+                if (j < c2) {
+                    m = rmass[s2 + j];
+                    y = ry[s2 + j];
+                    x = rx[s2 + j];
+                    take = m != 0.f && y >= bymin && y <= bymax && x >= bxmin && x <= bxmax;
+                }
+                const uint64_t tm = __ballot(take);
+                const uint32_t nt = (uint32_t)__popcll(tm);
+                if (S + nt > (uint32_t)DD_CAP) { staged = false; break; }
+                if (take) {
+                    const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
+                    const uint32_t l2 = slots[s2 + j];
+                    s_m[wave][r] = m;
+                    s_y[wave][r] = y;
+                    s_x[wave][r] = x;
+                    s_i[wave][r] = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
+                }
+                S += nt;
             }
-            bool keep = own && (m != 0.f);
-            if (__ballot(keep)) {
-                if (staged) {
-                    for (uint32_t j = 0; j < M; ++j) {
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    for (uint32_t ib = 0; ib < cnt; ib += 64) {
+        const uint32_t i = ib + lane;
+        const bool own = i < cnt;
+        float m = 0.f, y = 0.f, x = 0.f;
+        int32_t idx = -1;
+        if (own) {
+            m = rmass[slot0 + i];
+            y = ry[slot0 + i];
+            x = rx[slot0 + i];
+            const uint32_t l = slots[slot0 + i];
+            idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
+        }
+        const bool valid = own && (m != 0.f);
+        bool keep = valid;
+        if (staged) {
+            // one own candidate at a time, staged records across the lanes
+            uint64_t vm = __ballot(valid);
+            uint64_t dropped = 0;
+            while (vm) {
+                const int src = __ffsll((long long)vm) - 1;
+                vm &= vm - 1;
+                const float om = __shfl(m, src, 64), oy = __shfl(y, src, 64);
+                const float ox = __shfl(x, src, 64);
+                const int32_t oi = __shfl(idx, src, 64);
+                bool drop = false;
+                for (uint32_t jb = 0; jb < S; jb += 64) {
+                    const uint32_t j = jb + lane;
+                    if (j < S) {
                         const float nm = s_m[wave][j];
-                        if (nm == 0.f) continue;
-                        const float dy = s_y[wave][j] - y, dx = s_x[wave][j] - x;
+                        const float dy = s_y[wave][j] - oy, dx = s_x[wave][j] - ox;
                         float d = dy * dy;
                         d = d + dx * dx;
                         const int32_t nidx = s_i[wave][j];
-                        if (d < r2 && nidx != idx && (m < nm || (m == nm && idx > nidx)))
-                            keep = false;
+                        drop |= d < r2 && nidx != oi && (om < nm || (om == nm && oi > nidx));
                     }
-                } else {
-                    for (int ny = y_lo; ny <= y_hi; ++ny) {
-                        for (int nx = x_lo; nx <= x_hi; ++nx) {
-                            const uint32_t t2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-                            const uint32_t c2 = b.tile_count[t2];
-                            const size_t s2 = (size_t)t2 * TILE_PX;
-                            for (uint32_t j = 0; j < c2; ++j) {
-                                const float nm = rmass[s2 + j];
-                                if (nm == 0.f) continue;
-                                const float dy = ry[s2 + j] - y, dx = rx[s2 + j] - x;
-                                float d = dy * dy;
-                                d = d + dx * dx;
-                                if (d < r2 && !(t2 == item && j == i)) {
-                                    bool drop = m < nm;
-                                    if (!drop && m == nm) {
-                                        const uint32_t l2 = b.slots[s2 + j];
-                                        const int32_t nidx =
-                                            (ny * TILE_H + (int)(l2 / TILE_W)) * W +
-                                            (nx * TILE_W + (int)(l2 % TILE_W));
-                                        drop = idx > nidx;
-                                    }
-                                    if (drop) keep = false;
-                                }
+                }
+                if (__ballot(drop)) dropped |= 1ull << src;
+            }
+            keep = valid && !((dropped >> lane) & 1ull);
+        } else if (__ballot(keep)) {
+            for (int ny = y_lo; ny <= y_hi; ++ny) {
+                for (int nx = x_lo; nx <= x_hi; ++nx) {
+                    const uint32_t t2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+                    const uint32_t c2 = b.tile_count[t2];
+                    const size_t s2 = (size_t)t2 * TILE_PX;
+                    for (uint32_t j = 0; j < c2; ++j) {
+                        const float nm = rmass[s2 + j];
+                        if (nm == 0.f) continue;
+                        const float dy = ry[s2 + j] - y, dx = rx[s2 + j] - x;
+                        float d = dy * dy;
+                        d = d + dx * dx;
+                        if (d < r2 && !(t2 == item && j == i)) {
+                            bool drop = m < nm;
+                            if (!drop && m == nm) {
+                                const uint32_t l2 = slots[s2 + j];
+                                const int32_t nidx = (ny * TILE_H + (int)(l2 / TILE_W)) * W +
+                                                     (nx * TILE_W + (int)(l2 % TILE_W));
+                                drop = idx > nidx;
                             }
+                            if (drop) keep = false;
                         }
                     }
                 }
             }
-            if (own) b.keep[slot0 + i] = keep ? 1 : 0;
-            kept += (uint32_t)__popcll(__ballot(keep));
         }
-        if (lane == 0) b.tile_keep[item] = kept;
+        if (own) b.keep[slot0 + i] = keep ? 1 : 0;
+        kept += (uint32_t)__popcll(__ballot(keep));
     }
+    if (lane == 0) b.tile_keep[item] = kept;
 }
 
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
