@@ -75,6 +75,11 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
                     help="locate without the percentile test (the reference's default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="also run one device-to-device copy of MB megabytes (a known-size "
+                         "kernel for scaling hardware byte counters)")
+    ap.add_argument("--workload", choices=["movie", "uneven"], default="movie",
+                    help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -85,6 +90,8 @@ def main() -> int:
 
     from tracking_optimizations_amd import _lib, synth
     from tracking_optimizations_amd.gpu_tracking import make_params
+    if args.workload == "uneven":
+        return bench_uneven(args)
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
     if args.no_percentile:
@@ -119,12 +126,14 @@ def main() -> int:
 
     def sync():
         ctx.synchronize()
-        try:
-            import torch
-            if torch.cuda.is_available():
-                torch.cuda.synchronize()
-        except ImportError:
-            pass
+
+    if args.copy_calibration:
+        nb = min(args.copy_calibration << 20, movie.nbytes // 16 * 16)
+        dst = synth.DeviceBuffer(ctx, nb)
+        _lib.check(L.gt_memcpy_d2d(ctx.handle, ctypes.c_void_p(dst.ptr),
+                                   ctypes.c_void_p(movie.ptr), nb))
+        log(f"copy calibration: k_copy read {nb} bytes and wrote {nb} bytes")
+        dst.free()
 
     for i in range(args.warmup):
         t = run(batched_flags)
@@ -230,6 +239,68 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_uneven(args) -> int:
+    """Uneven batch: 4096 frames, 40 % 256x256, 40 % 512x1024, 20 % 2048x2048
+    (shuffled), one spot per 800 px^2, sigma U(1.2, 2.5), background U(50, 300);
+    batched vs plain per-frame path on the same device-resident frames."""
+    import ctypes
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import make_params
+    kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    if args.no_percentile:
+        del kw["percentile"]
+    params = make_params(**kw)
+    ctx = _lib.Context(0)
+    L = _lib.load()
+    nf = 4096 if args.frames == 1000 else args.frames
+    log(f"generating {nf} uneven frames on the device")
+    movie = synth.UnevenMovie(ctx, nf, seed=3)
+    descs = movie.frame_descs()
+    ctx.synchronize()
+
+    def run(flags):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_batch(ctx.handle, descs, nf, 0, ctypes.byref(params), flags,
+                                     ctypes.byref(out)))
+        r = _lib.Result(out)
+        t = ctx.timing()
+        del r
+        return t
+
+    bflags = _lib.GT_FRAMES_ON_DEVICE | _lib.GT_RESULT_ON_DEVICE
+    for _ in range(args.warmup):
+        run(bflags)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = run(bflags)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    fps = nf * args.steps / el
+    run(bflags | _lib.GT_PLAIN)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    run(bflags | _lib.GT_PLAIN)
+    ctx.synchronize()
+    pel = time.perf_counter() - t0
+    log(f"uneven batched: {fps:.0f} frames/s ({movie.pixels * args.steps / el / 1e9:.2f} Gpx/s); "
+        f"plain {nf / pel:.0f} frames/s; ratio {fps * pel / nf:.2f}x; "
+        f"{st['candidates']} candidates, {st['rows']} rows")
+    print(json.dumps({"metric": "frames/s, uneven batch (4096 frames: 40% 256^2, 40% 512x1024, "
+                                "20% 2048^2), batched vs plain",
+                      "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": 1000 * el / args.steps,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic (seeded device generator), device-resident",
+                      "config": {"workload": "uneven batch", "frames": nf, "params": kw,
+                                 "path": "batched"},
+                      "pixels_per_s": movie.pixels * args.steps / el,
+                      "plain_frames_per_s": nf / pel, "batched_over_plain": fps * pel / nf}),
+          flush=True)
+    movie.free()
     return 0
 
 

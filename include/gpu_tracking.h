@@ -223,6 +223,9 @@ gt_status gt_device_alloc(gt_context* ctx, uint64_t bytes, void** out);
 gt_status gt_device_free(gt_context* ctx, void* p);
 gt_status gt_memcpy_h2d(gt_context* ctx, void* dst, const void* src, uint64_t bytes);
 gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy by one plain copy kernel (16-byte aligned pointers);
+ * returns when the copy is done. */
+gt_status gt_memcpy_d2d(gt_context* ctx, void* dst, const void* src, uint64_t bytes);
 gt_status gt_synchronize(gt_context* ctx);
 
 #ifdef __cplusplus

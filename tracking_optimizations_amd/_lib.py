@@ -36,7 +36,7 @@ EXPORTS = [
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
-    "gt_memcpy_d2h", "gt_synchronize", "gt_context_set_profiling", "gt_context_stage_times",
+    "gt_memcpy_d2h", "gt_memcpy_d2d", "gt_synchronize", "gt_context_set_profiling", "gt_context_stage_times",
 ]
 
 STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
@@ -173,6 +173,7 @@ def load() -> ctypes.CDLL:
         L.gt_device_free.argtypes = [vp, vp]
         L.gt_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_uint64]
+        L.gt_memcpy_d2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_synchronize.argtypes = [vp]
         L.gt_context_set_profiling.argtypes = [vp, ctypes.c_int]
         L.gt_context_stage_times.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64)]

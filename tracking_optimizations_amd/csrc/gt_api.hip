@@ -270,7 +270,7 @@ struct gt_context {
     gt_timing last{};
     uint32_t batch_frames = 0;   // 0 = automatic
     size_t workspace_limit = (size_t)6 << 30;   // per workspace
-    int seg_rows = 64;
+    int seg_rows = 128;
     // per-stage event timing (forces one stream so that kernels do not overlap)
     int profiling = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -1065,6 +1065,17 @@ gt_status gt_memcpy_h2d(gt_context* ctx, void* dst, const void* src, uint64_t by
 gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t bytes) {
     if (!ctx) return fail(GT_E_INVALID, "NULL argument");
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+gt_status gt_memcpy_d2d(gt_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(GT_E_INVALID, "NULL argument");
+    if ((((uintptr_t)dst | (uintptr_t)src) & 15) != 0)
+        return fail(GT_E_INVALID, "gt_memcpy_d2d: pointers must be 16-byte aligned");
+    if (bytes == 0) return GT_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launch_copy(dst, src, bytes, ctx->streams[0]));
+    HIP_TRY(hipStreamSynchronize(ctx->streams[0]));
     return GT_OK;
 }
 

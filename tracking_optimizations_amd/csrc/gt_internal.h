@@ -133,6 +133,7 @@ hipError_t launch_frame_scan(const Batch& b, hipStream_t s);
 hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
                        hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
 
 // Synthetic frames (gt_synth.hip).
 hipError_t launch_synth(uint16_t* frames, uint64_t nframes, uint32_t h,

@@ -130,6 +130,27 @@ hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Plain device copy: 16-byte vectors, grid-stride, byte tail.  Also the kernel
+// of known traffic (n bytes read, n written) for scaling byte counters.
+__global__ void k_copy(uint4* __restrict__ dst, const uint4* __restrict__ src, uint64_t n16,
+                       uint8_t* __restrict__ dtail, const uint8_t* __restrict__ stail,
+                       uint32_t ntail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+        dst[i] = src[i];
+    if (blockIdx.x == 0 && threadIdx.x < ntail) dtail[threadIdx.x] = stail[threadIdx.x];
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+    const uint64_t n16 = bytes / 16;
+    const uint32_t ntail = (uint32_t)(bytes % 16);
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((n16 + 255) / 256, 8192);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src,
+                       n16, (uint8_t*)dst + n16 * 16, (const uint8_t*)src + n16 * 16, ntail);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // K1 bandpass, streaming column strips.
 //
@@ -365,12 +386,25 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
                         make_float4(out[0], out[1], out[2], out[3]);
                 }
                 if (kp.hist && cb + 3 < W) {
+                    // neighbouring pixels usually share a bin: merge runs of
+                    // equal keys before touching the LDS histogram
+                    uint32_t pk = 0xffffffffu, pc = 0;
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t bits = __float_as_uint(out[q]);
-                        zc += bits == 0 ? 1u : 0u;
-                        if (bits != 0) atomicAdd(&s_hist[pct_key(bits)], 1u);
+                        if (bits == 0) {
+                            ++zc;
+                        } else {
+                            const uint32_t key = pct_key(bits);
+                            if (key != pk) {
+                                if (pc) atomicAdd(&s_hist[pk], pc);
+                                pk = key;
+                                pc = 0;
+                            }
+                            ++pc;
+                        }
                     }
+                    if (pc) atomicAdd(&s_hist[pk], pc);
                 }
             }
             }  // vr < vr1
@@ -611,7 +645,6 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int LS = ((LC + 3) / 4) * 4 + 4;
     constexpr int NIN = N + KD - 1;
     constexpr int NV4 = (NIN + 3) / 4;
-    constexpr int NFILL = (LR * LC + 255) / 256;
     constexpr int NRUN = (LR * 4 + 255) / 256;
     __shared__ __attribute__((aligned(16))) float L[LR * LS];
     __shared__ uint64_t s_mask[TILE_H];
@@ -631,20 +664,30 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     if (tid == 0) s_lcnt = 0;
 
     {
-        float fill[NFILL];
+        // wave w stages rows w, w+4, ...: lane = column (plus a second pass
+        // over the KD-1 halo columns); all loads are issued before any store
+        constexpr int RPW = (LR + 3) / 4;
+        const int lane = tid & 63, wv = tid >> 6;
+        const int v1 = x0 - a + lane, v2 = v1 + 64;
+        const bool c1 = v1 >= 0 && v1 < W;
+        const bool c2 = (lane < LC - 64) && v2 >= 0 && v2 < W;
+        float f1[RPW], f2[RPW];
 #pragma unroll
-        for (int q = 0; q < NFILL; ++q) {
-            const int idx = tid + q * 256;
-            const int r = idx / LC, j = idx - r * LC;
-            const int u = y0 - a + r, v = x0 - a + j;
-            fill[q] = __uint_as_float(NEG_INF_BITS);
-            if (idx < LR * LC && u >= 0 && u < H && v >= 0 && v < W) fill[q] = P[(size_t)u * W + v];
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wv + 4 * q;
+            const int u = y0 - a + r;
+            const bool ur = r < LR && u >= 0 && u < H;
+            const GLOBAL float* prow = P + (size_t)(ur ? u : 0) * W;
+            f1[q] = (ur && c1) ? prow[v1] : __uint_as_float(NEG_INF_BITS);
+            f2[q] = (ur && c2) ? prow[v2] : __uint_as_float(NEG_INF_BITS);
         }
 #pragma unroll
-        for (int q = 0; q < NFILL; ++q) {
-            const int idx = tid + q * 256;
-            const int r = idx / LC, j = idx - r * LC;
-            if (idx < LR * LC) L[r * LS + j] = fill[q];
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wv + 4 * q;
+            if (r < LR) {
+                L[r * LS + lane] = f1[q];
+                if (lane < LC - 64) L[r * LS + 64 + lane] = f2[q];
+            }
         }
     }
     __syncthreads();
@@ -1028,6 +1071,103 @@ __device__ __forceinline__ void disk_any(const float* P, int H, int W, int band,
     else disk_eval(P, H, W, band, rint, u, v, m, c0, c1);
 }
 
+// Characterisation columns of one row, at the candidate (start) pixel:
+// execute_gpu.rs:165-172,286-299,346-352 (cpu_processed: Rg, ecc, signal
+// with the walked mass) and centers.wgsl:64-80 (characterize: Rg_gpu,
+// raw_mass_gpu, signal_gpu, ecc_gpu with the candidate pixel's own disk).
+__device__ __forceinline__ void characterise(const Batch& b, const Geom* __restrict__ g,
+                                          const float* __restrict__ P, const void* in, int H,
+                                          int W, int u0, int v0, float mass, size_t s) {
+    const uint64_t cap = b.slot_cap;
+    const int band = g->band, rint = g->rint;
+        if (g->cpu_processed) {
+            // execute_gpu.rs:165-172,286-299,346-352 at the candidate pixel:
+            // products with the host tables, each summed in ndarray's
+            // eight-accumulator order (full chunks of 8, pairwise fold, then
+            // the remainder in order).
+            const int r = g->band;
+            const int nt = g->ntap_cpu;
+            const int nfull = nt - (nt % 8);
+            float pr[8], ps[8], pc[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { pr[q] = 0.f; ps[q] = 0.f; pc[q] = 0.f; }
+            float sig = 0.f, mid = 0.f;
+            int tap = 0;
+            for (int di = -r; di <= r; ++di) {
+                for (int dj = -r; dj <= r; ++dj) {
+                    if (di * di + dj * dj > r * r) continue;
+                    const float d = P[(size_t)(u0 + di) * W + (v0 + dj)];
+                    sig = (tap == 0 || d > sig) ? d : sig;
+                    if (di == 0 && dj == 0) mid = d;
+                    if (tap < nfull) {
+                        const int q = tap & 7;
+                        pr[q] = pr[q] + d * g->r2_cpu[tap];
+                        ps[q] = ps[q] + d * g->sin_cpu[tap];
+                        pc[q] = pc[q] + d * g->cos_cpu[tap];
+                    }
+                    ++tap;
+                }
+            }
+            float rg = 0.f, ss = 0.f, cc = 0.f;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                rg = rg + (pr[q] + pr[q + 4]);
+                ss = ss + (ps[q] + ps[q + 4]);
+                cc = cc + (pc[q] + pc[q + 4]);
+            }
+            tap = 0;
+            for (int di = -r; di <= r; ++di) {
+                for (int dj = -r; dj <= r; ++dj) {
+                    if (di * di + dj * dj > r * r) continue;
+                    if (tap >= nfull) {
+                        const float d = P[(size_t)(u0 + di) * W + (v0 + dj)];
+                        rg = rg + d * g->r2_cpu[tap];
+                        ss = ss + d * g->sin_cpu[tap];
+                        cc = cc + d * g->cos_cpu[tap];
+                    }
+                    ++tap;
+                }
+            }
+            ss = ss * ss;
+            cc = cc * cc;
+            const float ecc = sqrtf(ss + cc);
+            b.rec[REC_RG * cap + s] = sqrtf(rg / mass);
+            b.rec[REC_ECC * cap + s] = ecc / (mass - mid);
+            b.rec[REC_SIGNAL * cap + s] = sig;
+        }
+        if (g->characterize) {
+            // centers.wgsl:64-80 at the candidate pixel
+            float o4[4] = {0.f, 0.f, 0.f, 0.f};
+            if (!(u0 - band < 0 || u0 + band >= H || v0 - band < 0 || v0 + band >= W)) {
+                float ms = 0.f, rg = 0.f, rawm = 0.f, sg = 0.f, es = 0.f, ec = 0.f;
+                int tap = 0;
+                for (int di = -rint; di <= rint; ++di)
+                    for (int dj = -rint; dj <= rint; ++dj) {
+                        if (di * di + dj * dj > rint * rint) continue;
+                        const size_t idx = (size_t)(u0 + di) * W + (v0 + dj);
+                        const float d = P[idx];
+                        ms = ms + d;
+                        rg = __builtin_fmaf(d, (float)(di * di + dj * dj), rg);
+                        rawm = rawm + raw_px(in, b.in_type, idx);
+                        sg = (d > sg) ? d : sg;
+                        es = __builtin_fmaf(d, g->sin2_gpu[tap], es);
+                        ec = __builtin_fmaf(d, g->cos2_gpu[tap], ec);
+                        ++tap;
+                    }
+                es = es * es;
+                ec = ec * ec;
+                o4[3] = sqrtf(es + ec) / (ms - P[(size_t)u0 * W + v0]);
+                o4[0] = sqrtf(rg / ms);
+                o4[1] = rawm;
+                o4[2] = sg;
+            }
+            b.rec[REC_RG_GPU * cap + s] = o4[0];
+            b.rec[REC_RAW_GPU * cap + s] = o4[1];
+            b.rec[REC_SIG_GPU * cap + s] = o4[2];
+            b.rec[REC_ECC_GPU * cap + s] = o4[3];
+        }
+}
+
 // Candidates of RT_TILES consecutive tiles are packed over the 256 lanes of a
 // workgroup (a tile holds ~16 candidates on average; one wave per tile would
 // leave three quarters of the lanes idle while issuing the same number of
@@ -1124,106 +1264,195 @@ __global__ __launch_bounds__(256) void k_refine(Batch b, const Geom* __restrict_
         rec[REC_MASS * cap + slot0 + i] = valid ? mass : 0.f;
         rec[REC_Y * cap + slot0 + i] = y;
         rec[REC_X * cap + slot0 + i] = x;
-        if (valid && g->cpu_processed) {
-            // execute_gpu.rs:165-172,286-299,346-352 at the candidate pixel:
-            // products with the host tables, each summed in ndarray's
-            // eight-accumulator order (full chunks of 8, pairwise fold, then
-            // the remainder in order).
-            const int r = g->band;
-            const int nt = g->ntap_cpu;
-            const int nfull = nt - (nt % 8);
-            float pr[8], ps[8], pc[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) { pr[q] = 0.f; ps[q] = 0.f; pc[q] = 0.f; }
-            float sig = 0.f, mid = 0.f;
-            int tap = 0;
-            for (int di = -r; di <= r; ++di) {
-                for (int dj = -r; dj <= r; ++dj) {
-                    if (di * di + dj * dj > r * r) continue;
-                    const float d = P[(size_t)(u0 + di) * W + (v0 + dj)];
-                    sig = (tap == 0 || d > sig) ? d : sig;
-                    if (di == 0 && dj == 0) mid = d;
-                    if (tap < nfull) {
-                        const int q = tap & 7;
-                        pr[q] = pr[q] + d * g->r2_cpu[tap];
-                        ps[q] = ps[q] + d * g->sin_cpu[tap];
-                        pc[q] = pc[q] + d * g->cos_cpu[tap];
-                    }
-                    ++tap;
-                }
-            }
-            float rg = 0.f, ss = 0.f, cc = 0.f;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                rg = rg + (pr[q] + pr[q + 4]);
-                ss = ss + (ps[q] + ps[q + 4]);
-                cc = cc + (pc[q] + pc[q + 4]);
-            }
-            tap = 0;
-            for (int di = -r; di <= r; ++di) {
-                for (int dj = -r; dj <= r; ++dj) {
-                    if (di * di + dj * dj > r * r) continue;
-                    if (tap >= nfull) {
-                        const float d = P[(size_t)(u0 + di) * W + (v0 + dj)];
-                        rg = rg + d * g->r2_cpu[tap];
-                        ss = ss + d * g->sin_cpu[tap];
-                        cc = cc + d * g->cos_cpu[tap];
-                    }
-                    ++tap;
-                }
-            }
-            ss = ss * ss;
-            cc = cc * cc;
-            const float ecc = sqrtf(ss + cc);
-            rec[REC_RG * cap + slot0 + i] = sqrtf(rg / mass);
-            rec[REC_ECC * cap + slot0 + i] = ecc / (mass - mid);
-            rec[REC_SIGNAL * cap + slot0 + i] = sig;
+        if (valid && (g->cpu_processed || g->characterize))
+            characterise(b, g, P, rt.in, H, W, u0, v0, mass, slot0 + i);
+    }
+}
+
+// Refinement with LDS-staged disk boxes (1 <= RI <= 7, max_iterations >= 1).
+// Lanes hold candidates (16 consecutive tiles per workgroup).  Every walk
+// step needs the disk around the lane's current pixel; instead of each lane
+// issuing (2RI+1)^2 scattered 4-byte loads, the wave stages the
+// (2RI+1)x(2RI+1) boxes of up to 16 lanes at a time with row-coalesced loads
+// (two load instructions per box for RI = 5), then each owner lane sums its
+// disk from LDS in the shader's tap order.
+template <int RI>
+__global__ __launch_bounds__(256) void k_refine_staged(Batch b, const Geom* __restrict__ g) {
+    constexpr int BW = 2 * RI + 1;
+    constexpr int BOX = BW * BW;
+    constexpr int G = 16;
+    constexpr int NE = (BOX + 63) / 64;
+    __shared__ RefTile s_t[RT_TILES];
+    __shared__ uint32_t s_total;
+    __shared__ float s_box[4][G][BOX];
+    const uint32_t item0 = blockIdx.x * RT_TILES;
+    const int nt = (int)min<uint32_t>(RT_TILES, b.tiles - item0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 64) {
+        uint32_t cnt = 0;
+        RefTile rt;
+        if (tid < nt) {
+            const uint32_t item = item0 + tid;
+            const uint32_t f = b.tile_frame[item];
+            const Frame fr = b.frames[f];
+            const uint32_t local = item - fr.tile_base;
+            rt.P = fr.P;
+            rt.in = fr.in;
+            rt.x0 = (int)(local % fr.tx) * TILE_W;
+            rt.y0 = (int)(local / fr.tx) * TILE_H;
+            rt.H = (int)fr.H;
+            rt.W = (int)fr.W;
+            rt.f = f;
+            cnt = b.tile_count[item];
+            rt.cnt = cnt;
         }
-        if (valid && g->characterize) {
-            // centers.wgsl:64-80 at the candidate pixel
-            float o4[4] = {0.f, 0.f, 0.f, 0.f};
-            if (!(u0 - band < 0 || u0 + band >= H || v0 - band < 0 || v0 + band >= W)) {
-                float ms = 0.f, rg = 0.f, rawm = 0.f, sg = 0.f, es = 0.f, ec = 0.f;
-                int tap = 0;
-                for (int di = -rint; di <= rint; ++di)
-                    for (int dj = -rint; dj <= rint; ++dj) {
-                        if (di * di + dj * dj > rint * rint) continue;
-                        const size_t idx = (size_t)(u0 + di) * W + (v0 + dj);
-                        const float d = P[idx];
-                        ms = ms + d;
-                        rg = __builtin_fmaf(d, (float)(di * di + dj * dj), rg);
-                        rawm = rawm + raw_px(rt.in, b.in_type, idx);
-                        sg = (d > sg) ? d : sg;
-                        es = __builtin_fmaf(d, g->sin2_gpu[tap], es);
-                        ec = __builtin_fmaf(d, g->cos2_gpu[tap], ec);
-                        ++tap;
+        const uint32_t inc = wave_incl_scan(cnt, tid);
+        if (tid < nt) {
+            rt.pref = inc - cnt;
+            s_t[tid] = rt;
+        }
+        if (tid == 63) s_total = inc;
+    }
+    __syncthreads();
+    const uint32_t total = s_total;
+    const int band = g->band;
+    const uint32_t maxit = g->max_iterations;
+    const float minmass = g->minmass;
+    const int has_pct = g->has_pct;
+    const float sthr = 0.6f;
+    const uint64_t cap = b.slot_cap;
+
+    for (uint32_t gb = 0; gb < total; gb += 256) {
+        const uint32_t gi = gb + tid;
+        const bool has = gi < total;
+        int j = 0;
+#pragma unroll
+        for (int q = 1; q < RT_TILES; ++q)
+            if (q < nt && s_t[q].pref <= gi) j = q;
+        const RefTile& rt = s_t[j];
+        const uint32_t i = has ? gi - rt.pref : 0;
+        const size_t slot = (size_t)(item0 + (uint32_t)j) * TILE_PX + i;
+        const int H = rt.H, W = rt.W;
+        int u0 = 0, v0 = 0;
+        bool active = false;
+        if (has) {
+            const uint32_t l = b.slots[slot];
+            u0 = rt.y0 + (int)(l / TILE_W);
+            v0 = rt.x0 + (int)(l % TILE_W);
+            active = true;
+            if (has_pct) active = (double)rt.P[(size_t)u0 * W + v0] > b.thr[rt.f];
+        }
+        const bool started = active;
+        int u = u0, v = v0;
+        float c0 = 0.f, c1 = 0.f, m = 0.f;
+        uint32_t it = 0;
+        while (__ballot(active)) {
+            uint64_t pend = __ballot(active);
+            while (pend) {
+                int src[G];
+                uint64_t gm = 0;
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    src[q] = pend ? __ffsll((long long)pend) - 1 : -1;
+                    if (pend) {
+                        gm |= 1ull << src[q];
+                        pend &= pend - 1;
                     }
-                es = es * es;
-                ec = ec * ec;
-                o4[3] = sqrtf(es + ec) / (ms - P[(size_t)u0 * W + v0]);
-                o4[0] = sqrtf(rg / ms);
-                o4[1] = rawm;
-                o4[2] = sg;
+                }
+                float fb[G][NE];
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    if (src[q] >= 0) {
+                        const int uq = __shfl(u, src[q], 64), vq = __shfl(v, src[q], 64);
+                        const int jq = __shfl(j, src[q], 64);
+                        const RefTile& tq = s_t[jq];
+                        if (uq - band >= 0 && uq + band < tq.H && vq - band >= 0 && vq + band < tq.W) {
+                            const GLOBAL float* Pq = (const GLOBAL float*)tq.P;
+#pragma unroll
+                            for (int e = 0; e < NE; ++e) {
+                                const int el = lane + 64 * e;
+                                if (el < BOX) {
+                                    const int r = el / BW, c = el - r * BW;
+                                    fb[q][e] = Pq[(size_t)(uq - RI + r) * tq.W + (vq - RI + c)];
+                                }
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < G; ++q) {
+                    if (src[q] >= 0) {
+#pragma unroll
+                        for (int e = 0; e < NE; ++e) {
+                            const int el = lane + 64 * e;
+                            if (el < BOX) s_box[wave][q][el] = fb[q][e];
+                        }
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if ((gm >> lane) & 1ull) {
+                    const int sl = (int)__popcll(gm & lanemask_lt(lane));
+                    if (u - band < 0 || u + band >= H || v - band < 0 || v + band >= W) {
+                        m = 0.f; c0 = 0.f; c1 = 0.f;
+                    } else {
+                        const float* box = s_box[wave][sl];
+                        float s0 = 0.f, s1 = 0.f, ms = 0.f;
+#pragma unroll
+                        for (int di = -RI; di <= RI; ++di)
+#pragma unroll
+                            for (int dj = -RI; dj <= RI; ++dj) {
+                                if (di * di + dj * dj > RI * RI) continue;
+                                const float d = box[(di + RI) * BW + (dj + RI)];
+                                s0 = __builtin_fmaf(d, (float)di, s0);
+                                s1 = __builtin_fmaf(d, (float)dj, s1);
+                                ms = ms + d;
+                            }
+                        m = ms;
+                        c0 = s0 / ms;
+                        c1 = s1 / ms;
+                    }
+                    bool changed = false;
+                    if (c0 > sthr && u < H - 1) { u += 1; changed = true; }
+                    else if (c0 < -sthr && u > 0) { u -= 1; changed = true; }
+                    if (c1 > sthr && v < W - 1) { v += 1; changed = true; }
+                    else if (c1 < -sthr && v > 0) { v -= 1; changed = true; }
+                    ++it;
+                    if (!changed || it >= maxit) active = false;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
-            rec[REC_RG_GPU * cap + slot0 + i] = o4[0];
-            rec[REC_RAW_GPU * cap + slot0 + i] = o4[1];
-            rec[REC_SIG_GPU * cap + slot0 + i] = o4[2];
-            rec[REC_ECC_GPU * cap + slot0 + i] = o4[3];
+        }
+        if (has) {
+            const float y = (float)u + c0, x = (float)v + c1;
+            const bool valid = started && (m > minmass) && (m != 0.f);
+            b.rec[REC_MASS * cap + slot] = valid ? m : 0.f;
+            b.rec[REC_Y * cap + slot] = started ? y : 0.f;
+            b.rec[REC_X * cap + slot] = started ? x : 0.f;
+            if (valid && (g->cpu_processed || g->characterize))
+                characterise(b, g, rt.P, rt.in, H, W, u0, v0, m, slot);
         }
     }
 }
 
 hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
     const dim3 grid((b.tiles + RT_TILES - 1) / RT_TILES);
-    switch (g.rint) {
-        case 2: hipLaunchKernelGGL(k_refine<2>, grid, dim3(256), 0, s, b, d_geom); break;
-        case 3: hipLaunchKernelGGL(k_refine<3>, grid, dim3(256), 0, s, b, d_geom); break;
-        case 4: hipLaunchKernelGGL(k_refine<4>, grid, dim3(256), 0, s, b, d_geom); break;
-        case 5: hipLaunchKernelGGL(k_refine<5>, grid, dim3(256), 0, s, b, d_geom); break;
-        case 6: hipLaunchKernelGGL(k_refine<6>, grid, dim3(256), 0, s, b, d_geom); break;
-        case 7: hipLaunchKernelGGL(k_refine<7>, grid, dim3(256), 0, s, b, d_geom); break;
-        default: hipLaunchKernelGGL(k_refine<0>, grid, dim3(256), 0, s, b, d_geom); break;
+    if (g.max_iterations >= 1) {
+        switch (g.rint) {
+            case 1: hipLaunchKernelGGL(k_refine_staged<1>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 2: hipLaunchKernelGGL(k_refine_staged<2>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 3: hipLaunchKernelGGL(k_refine_staged<3>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 4: hipLaunchKernelGGL(k_refine_staged<4>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 5: hipLaunchKernelGGL(k_refine_staged<5>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 6: hipLaunchKernelGGL(k_refine_staged<6>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            case 7: hipLaunchKernelGGL(k_refine_staged<7>, grid, dim3(256), 0, s, b, d_geom); return hipGetLastError();
+            default: break;
+        }
     }
+    hipLaunchKernelGGL(k_refine<0>, grid, dim3(256), 0, s, b, d_geom);
     return hipGetLastError();
 }
 

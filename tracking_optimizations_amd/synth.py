@@ -173,3 +173,44 @@ def generate_host(ctx: "_lib.Context", spec: MovieSpec) -> np.ndarray:
         return buf.to_host(np.uint16, (spec.n_frames, spec.h, spec.w))
     finally:
         buf.free()
+
+
+class UnevenMovie:
+    """Device-resident uneven batch: frames of each shape live in one device
+    buffer per shape; `descs` lists (ptr, h, w) in the shuffled frame order."""
+
+    def __init__(self, ctx: "_lib.Context", n_frames: int = 4096, seed: int = 3):
+        shapes = uneven_shapes(seed, n_frames)
+        rng = np.random.default_rng(seed + 1)
+        by_shape: dict = {}
+        for i, s in enumerate(shapes):
+            by_shape.setdefault(s, []).append(i)
+        self.buffers = []
+        self.descs = [None] * n_frames
+        self.pixels = 0
+        for (h, w), idxs in sorted(by_shape.items()):
+            specs = [uneven_frame_spec((h, w), rng) for _ in idxs]
+            spots = np.concatenate([s.spots for s in specs]).astype(np.float32)
+            counts = [len(s.spots) for s in specs]
+            off = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+            bg = np.concatenate([s.bg for s in specs]).astype(np.float32)
+            spec = MovieSpec(h, w, spots, off, bg, seed * 1000 + h * 7 + w)
+            buf = generate(ctx, spec)
+            self.buffers.append(buf)
+            for k, fi in enumerate(idxs):
+                self.descs[fi] = (buf.ptr + k * h * w * 2, h, w)
+                self.pixels += h * w
+        self.shapes = shapes
+
+    def frame_descs(self):
+        arr = (_lib.GtFrameDesc * len(self.descs))()
+        for i, (p, h, w) in enumerate(self.descs):
+            arr[i].data = p
+            arr[i].h = h
+            arr[i].w = w
+            arr[i].frame_index = i
+        return arr
+
+    def free(self) -> None:
+        for b in self.buffers:
+            b.free()
