@@ -144,6 +144,10 @@ gt_status gt_context_destroy(gt_context* ctx);
 gt_status gt_context_timing(const gt_context* ctx, gt_timing* out);
 /* Frames per launch of the batched path (0 = automatic). */
 gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch);
+/* 1 (default): where the geometry allows it, the centroid walk runs inside
+ * the detect kernel on the tile it has staged; 0: a separate refine kernel
+ * walks every candidate.  Results are identical either way. */
+gt_status gt_context_set_fused_walk(gt_context* ctx, int enable);
 
 /* Per-stage kernel timing (HIP events recorded around every launch on the
  * context's stream).  Stages, in pipeline order: */

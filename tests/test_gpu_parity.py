@@ -47,6 +47,9 @@ CASES = {
     "wide_sep": dict(diameter=9, separation=20),
     "pct0": dict(diameter=9, separation=9, percentile=0),
     "pct100": dict(diameter=9, separation=9, percentile=100),
+    # dilation halo (3) narrower than the disk radius (5): walks near tile
+    # edges read their disks from P instead of the staged tile
+    "halo_fallback": dict(diameter=11, separation=5, smoothing_size=9, percentile=50),
 }
 
 
@@ -162,3 +165,18 @@ def test_multi_batch_two_streams_identical(frames512, per_launch):
     ctx.set_batch(per_launch)
     got = locate(frames512, p, ctx=ctx)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "defaults", "even_d", "iter1", "small_sep",
+                                  "wide_sep", "pct0", "pct100", "halo_fallback"])
+def test_walk_in_detect_equals_refine_kernel(frames512, case):
+    """The centroid walk fused into detect and the separate refine kernel
+    give identical rows."""
+    p = make_params(**CASES[case])
+    fused = _lib.Context(0)
+    separate = _lib.Context(0)
+    separate.set_fused_walk(False)
+    print(f"[parity] fused walk vs refine kernel case={case}", flush=True)
+    a = locate(frames512, p, ctx=fused)
+    b = locate(frames512, p, ctx=separate)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

@@ -32,7 +32,8 @@ DTYPE_CODES = {np.dtype(np.uint16): 0, np.dtype(np.float32): 1, np.dtype(np.uint
 EXPORTS = [
     "gt_params_default", "gt_params_rust_default", "gt_params_validate", "gt_column_names",
     "gt_last_error", "gt_version", "gt_device_count", "gt_context_create",
-    "gt_context_destroy", "gt_context_timing", "gt_context_set_batch", "gt_locate_u16",
+    "gt_context_destroy", "gt_context_timing", "gt_context_set_batch",
+    "gt_context_set_fused_walk", "gt_locate_u16",
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
@@ -146,6 +147,7 @@ def load() -> ctypes.CDLL:
         L.gt_context_destroy.argtypes = [vp]
         L.gt_context_timing.argtypes = [vp, P(GtTiming)]
         L.gt_context_set_batch.argtypes = [vp, ctypes.c_uint32]
+        L.gt_context_set_fused_walk.argtypes = [vp, ctypes.c_int]
         for name in ("gt_locate_u16", "gt_locate_u8", "gt_locate_f32"):
             f = getattr(L, name)
             f.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(GtParams),
@@ -230,6 +232,9 @@ class Context:
 
     def set_batch(self, frames_per_launch: int) -> None:
         check(load().gt_context_set_batch(self.handle, frames_per_launch))
+
+    def set_fused_walk(self, enable: bool) -> None:
+        check(load().gt_context_set_fused_walk(self.handle, 1 if enable else 0))
 
     def synchronize(self) -> None:
         check(load().gt_synchronize(self.handle))

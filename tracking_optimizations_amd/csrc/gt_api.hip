@@ -271,6 +271,7 @@ struct gt_context {
     uint32_t batch_frames = 0;   // 0 = automatic
     size_t workspace_limit = (size_t)6 << 30;   // per workspace
     int seg_rows = 128;
+    int fuse_walk = 1;     // run the centroid walk inside detect where supported
     // per-stage event timing (forces one stream so that kernels do not overlap)
     int profiling = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -327,6 +328,7 @@ FrameSizes sizes_of(uint32_t h, uint32_t w, int seg) {
 int rec_planes(const Geom& g) {
     if (g.characterize) return REC_NPLANES;
     if (g.cpu_processed) return REC_SIGNAL + 1;
+    if (g.walk_in_detect && g.has_pct) return REC_CENTER + 1;
     return REC_X + 1;
 }
 
@@ -407,6 +409,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
 
     Geom g;
     GT_TRY(make_geom(p, integer_input, g));
+    g.walk_in_detect = ctx->fuse_walk && walk_in_detect_supported(g) ? 1 : 0;
     const int seg = ctx->seg_rows;
     const size_t es = elem_size(in_type);
     // vector loads need W % 4 == 0 and frame bases aligned to 4 pixels
@@ -478,7 +481,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     ctx->last.pixels = total_px;
 
     // ---- workspaces -------------------------------------------------------
-    const int nstreams = (plain || ctx->profiling || plan.size() < 2) ? 1 : kStreams;
+    const int nstreams = (plain || plan.size() < 2) ? 1 : kStreams;
     const uint64_t slot_cap = maxneed.tiles * TILE_PX;
     const size_t nfb = maxneed.nframes;
     // per-frame: hist (4096 u32), frame_rows, frame_cands, spare, selstate(4), thr(double), row_base(u64)
@@ -906,6 +909,12 @@ gt_status gt_context_stage_times(const gt_context* ctx, double* ms, uint64_t* la
         if (ms) ms[i] = ctx->stage_ms[i];
         if (launches) launches[i] = ctx->stage_launches[i];
     }
+    return GT_OK;
+}
+
+gt_status gt_context_set_fused_walk(gt_context* ctx, int enable) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    ctx->fuse_walk = enable ? 1 : 0;
     return GT_OK;
 }
 

@@ -15,7 +15,7 @@ constexpr int TILE_W = 64;
 constexpr int TILE_PX = TILE_H * TILE_W;
 
 // Bandpass (K1) column strip: 64 lanes x 4 columns.
-constexpr int K1_STRIP = 256;
+constexpr int K1_STRIP = 240;   // output columns per bandpass work item
 
 // Percentile histogram: 4096 bins over the float32 bit pattern
 // (bin 0 = exact zero, bin b >= 1 = bits >> 19 == b - 1).
@@ -28,7 +28,10 @@ enum RecPlane : int {
     REC_MASS = 0, REC_Y = 1, REC_X = 2,
     REC_RG = 3, REC_ECC = 4, REC_SIGNAL = 5,                 // cpu_processed
     REC_RG_GPU = 6, REC_RAW_GPU = 7, REC_SIG_GPU = 8, REC_ECC_GPU = 9,  // characterize
-    REC_NPLANES = 10
+    REC_NPLANES = 10,
+    // start value P[u0, v0] of each candidate, stored only when the walk
+    // runs in detect (characterisation off, so this plane is free)
+    REC_CENTER = 3
 };
 
 struct Frame {
@@ -69,6 +72,7 @@ struct Geom {
     int32_t ncols;         // output columns
     int32_t ntap_gpu;      // disk taps (characterize tables)
     int32_t ntap_cpu;      // circle taps (cpu_processed tables)
+    int32_t walk_in_detect;  // centroid walk fused into detect (no refine launch)
     float sin2_gpu[1024], cos2_gpu[1024];                // disk radius rint
     float r2_cpu[1024], sin_cpu[1024], cos_cpu[1024];    // circle radius band
 };
@@ -121,6 +125,7 @@ hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom,
 hipError_t launch_bandpass_generic(const Batch& b, const Geom* d_geom, uint32_t max_px,
                                    int hist, hipStream_t s);
 hipError_t launch_select1(const Batch& b, const Geom& g, hipStream_t s);
+bool walk_in_detect_supported(const Geom& g);
 hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_select2(const Batch& b, const Geom& g, hipStream_t s);

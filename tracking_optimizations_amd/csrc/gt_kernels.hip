@@ -177,8 +177,10 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 // address space.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define GLOBAL __attribute__((address_space(1)))
+#define SHARED __attribute__((address_space(3)))
 #else
 #define GLOBAL
+#define SHARED
 #endif
 
 struct K1Params {
@@ -200,17 +202,18 @@ __device__ __forceinline__ Chunk4<T> ld_chunk(const GLOBAL T* row, int src) {
     return c;
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 template <typename T>
-__device__ __forceinline__ float4 unpack_chunk(const Chunk4<T>& c, bool rev) {
-    float4 r;
+__device__ __forceinline__ f4v unpack_chunk(const Chunk4<T>& c) {
+    f4v r;
     if constexpr (sizeof(T) == 2) {
-        r = make_float4((float)(c.v.x & 0xffffu), (float)(c.v.x >> 16),
-                        (float)(c.v.y & 0xffffu), (float)(c.v.y >> 16));
+        r.x = (float)(c.v.x & 0xffffu); r.y = (float)(c.v.x >> 16);
+        r.z = (float)(c.v.y & 0xffffu); r.w = (float)(c.v.y >> 16);
     } else {
-        r = make_float4((float)(c.v & 0xffu), (float)((c.v >> 8) & 0xffu),
-                        (float)((c.v >> 16) & 0xffu), (float)(c.v >> 24));
+        r.x = (float)(c.v & 0xffu); r.y = (float)((c.v >> 8) & 0xffu);
+        r.z = (float)((c.v >> 16) & 0xffu); r.w = (float)(c.v >> 24);
     }
-    if (rev) r = make_float4(r.w, r.z, r.y, r.x);
     return r;
 }
 
@@ -233,13 +236,39 @@ __device__ __forceinline__ float cdiv_t(float x, float d, float rd, bool fast) {
     return x / d;
 }
 
+// Five 16-byte LDS reads kept whole (volatile): otherwise the compiler
+// drops unused lanes and splits them into 4-byte reads, which at a 16-byte
+// lane stride collide on the LDS banks.
+__device__ __forceinline__ void lds_ld20(const float* p, float (&v)[20]) {
+    const volatile SHARED f4v* q = (const volatile SHARED f4v*)p;
+    const f4v a0 = q[0], a1 = q[1], a2 = q[2], a3 = q[3], a4 = q[4];
+    v[0] = a0.x;  v[1] = a0.y;  v[2] = a0.z;  v[3] = a0.w;
+    v[4] = a1.x;  v[5] = a1.y;  v[6] = a1.z;  v[7] = a1.w;
+    v[8] = a2.x;  v[9] = a2.y;  v[10] = a2.z; v[11] = a2.w;
+    v[12] = a3.x; v[13] = a3.y; v[14] = a3.z; v[15] = a3.w;
+    v[16] = a4.x; v[17] = a4.y; v[18] = a4.z; v[19] = a4.w;
+}
+
+// K1 layout: lane t owns the 4-column chunk at column x0 - 8 + 4t; lanes
+// 2..61 produce output (a 240-column strip), lanes 0, 1, 62, 63 only load
+// the 8-column halos on either side.
+constexpr int K1_LANE0 = 2;
+constexpr int K1_LANES = K1_STRIP / 4;
+static_assert(K1_LANES + 2 * K1_LANE0 == 64, "strip + halo must fill one wave");
+constexpr int K1_ROWBUF = 256;   // floats per staged row
+
+// Three waves per SIMD fit without spills up to R = 5.
 template <int R, typename T, bool FAST>
-__global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
+                          amdgpu_waves_per_eu(R <= 5 ? 3 : 2)))
+void k_bandpass(Batch b, K1Params kp) {
     constexpr int NT = 2 * R + 1;
-    constexpr int LB = K1_STRIP + 16;
-    __shared__ __attribute__((aligned(16))) float bufg[2][LB];
-    __shared__ __attribute__((aligned(16))) float bufb[2][LB];
-    __shared__ uint32_t s_hist[HIST_BINS];
+    static_assert(R <= 8, "taps must stay inside the staged halo");
+    // per parity: boxcar copy (mirrored) then Gaussian copy (zero outside)
+    __shared__ __attribute__((aligned(16))) float buf[2][2][K1_ROWBUF];
+    // two 16-bit bins per word; a work item adds at most
+    // K1_STRIP * seg < 65536 to one bin (checked by the host)
+    __shared__ uint32_t s_hist[HIST_BINS / 2 + 64];
 
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t f = b.k1_frame[item];
@@ -253,19 +282,18 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     const int t = threadIdx.x;
     const GLOBAL T* in = (const GLOBAL T*)fr.in;
     GLOBAL float* P = (GLOBAL float*)fr.P;
-    // strip touches a frame edge (halo or main chunks outside the frame)
+    // some chunk of this strip (output or halo) lies outside the frame
     const bool border = (x0 - 8 < 0) || (x0 + K1_STRIP + 8 > W);
-    const int cb = x0 + 4 * t;
-    // main chunk of this lane and (lanes 0..3) one halo chunk
-    int msrc, hsrc;
-    bool mrev, moob, hrev, hoob;
-    chunk_src(cb, W, msrc, mrev, moob);
-    const int hc = (t < 2) ? (x0 - 8 + 4 * t) : (x0 + K1_STRIP + 4 * (t - 2));
-    const int hoff = (t < 2) ? (4 * t) : (8 + K1_STRIP + 4 * (t - 2));
-    chunk_src(hc, W, hsrc, hrev, hoob);
+    const int cc = x0 - 8 + 4 * t;
+    int csrc;
+    bool crev, coob;
+    chunk_src(cc, W, csrc, crev, coob);
+    const bool outl = t >= K1_LANE0 && t < K1_LANE0 + K1_LANES && cc < W;
+    // first staged column this lane reads (halo lanes read in range, unused)
+    const int rbase = min(max(4 * t - 8, 0), K1_ROWBUF - 20);
 
     if (kp.hist) {
-        for (int i = t; i < HIST_BINS; i += 64) s_hist[i] = 0;
+        for (int i = t; i < HIST_BINS / 2 + 64; i += 64) s_hist[i] = 0;
     }
 
     float w[NT];
@@ -278,7 +306,6 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) { rg[s][q] = 0.f; rb[s][q] = 0.f; }
     float vbox[4] = {0.f, 0.f, 0.f, 0.f};
-    uint32_t zc = 0;
 
     const int vr0 = y0 - R, vr1 = y1 + R;
     auto row_ptr = [&](int vr) {
@@ -286,15 +313,13 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
         return in + (size_t)ar * (size_t)W;
     };
 
-    // prefetch ring: row vr lives in slot (vr - vr0) % NT until it is staged
-    Chunk4<T> pm[NT], ph[NT];
+    // prefetch: row vr is loaded PF rows ahead into slot (vr - vr0) % NT,
+    // so only PF + 1 slots are live at a time
+    constexpr int PF = NT < 4 ? NT : 4;
+    Chunk4<T> pm[NT];
 #pragma unroll
-    for (int s = 0; s < NT; ++s) {
-        if (vr0 + s < vr1) {
-            const GLOBAL T* row = row_ptr(vr0 + s);
-            pm[s] = ld_chunk<T>(row, msrc);
-            if (t < 4) ph[s] = ld_chunk<T>(row, hsrc);
-        }
+    for (int s = 0; s < PF; ++s) {
+        if (vr0 + s < vr1) pm[s] = ld_chunk<T>(row_ptr(vr0 + s), csrc);
     }
 
     int parity = 0;
@@ -303,66 +328,57 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
         for (int s = 0; s < NT; ++s) {
             const int vr = base + s;
             if (vr < vr1) {  // wave-uniform
-            float* bg_ = bufg[parity];
-            float* bb_ = bufb[parity];
+            float* bb_ = buf[parity][0];
+            float* bg_ = buf[parity][1];
             parity ^= 1;
+            // rows above and below the frame are zero in the Gaussian copy
+            // (skipped taps) and mirrored in the boxcar copy
+            const bool row_in = (vr >= 0) && (vr < H);
             {
-                const float4 m = unpack_chunk<T>(pm[s], mrev);
-                *(float4*)(bg_ + 8 + 4 * t) = moob ? make_float4(0.f, 0.f, 0.f, 0.f) : m;
-                *(float4*)(bb_ + 8 + 4 * t) = m;
-                if (t < 4) {
-                    const float4 h = unpack_chunk<T>(ph[s], hrev);
-                    *(float4*)(bg_ + hoff) = hoob ? make_float4(0.f, 0.f, 0.f, 0.f) : h;
-                    *(float4*)(bb_ + hoff) = h;
+                f4v m = unpack_chunk<T>(pm[s]);
+                if (border || !row_in) {
+                    if (crev) m = m.wzyx;
+                    *(f4v*)(bb_ + 4 * t) = m;
+                    if (coob || !row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
+                    *(f4v*)(bg_ + 4 * t) = m;
+                } else {
+                    *(f4v*)(bb_ + 4 * t) = m;
+                    *(f4v*)(bg_ + 4 * t) = m;
                 }
             }
-            if (vr + NT < vr1) {
-                const GLOBAL T* row = row_ptr(vr + NT);
-                pm[s] = ld_chunk<T>(row, msrc);
-                if (t < 4) ph[s] = ld_chunk<T>(row, hsrc);
-            }
+            if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(row_ptr(vr + PF), csrc);
             __syncthreads();
 
+            // boxcar row pass on the mirrored copy (integer sums: exact)
             float v[20];
+            lds_ld20(bb_ + rbase, v);
+            float tb[4];
+            {
+                float bx = v[8 - R];
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const float4 x = *(const float4*)(bg_ + 4 * t + 4 * m);
-                v[4 * m + 0] = x.x; v[4 * m + 1] = x.y; v[4 * m + 2] = x.z; v[4 * m + 3] = x.w;
+                for (int i = -R + 1; i <= R; ++i) bx = bx + v[8 + i];
+                tb[0] = bx;
+#pragma unroll
+                for (int q = 1; q < 4; ++q) {
+                    bx = (bx + v[8 + q + R]) - v[8 + q - 1 - R];
+                    tb[q] = bx;
+                }
             }
-            float tg[4], tb[4];
+            // Gaussian row pass: zeros outside the frame (skipped taps); the
+            // second copy is read in every strip so no registers are merged
+            float u[20];
+            lds_ld20(bg_ + rbase, u);
+            float tg[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 float gs = 0.f;
 #pragma unroll
-                for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], v[8 + q + i], gs);
+                for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
                 tg[q] = gs;
             }
-            float vb[20];
-            if (border) {
-#pragma unroll
-                for (int m = 0; m < 5; ++m) {
-                    const float4 x = *(const float4*)(bb_ + 4 * t + 4 * m);
-                    vb[4 * m + 0] = x.x; vb[4 * m + 1] = x.y; vb[4 * m + 2] = x.z; vb[4 * m + 3] = x.w;
-                }
-            } else {
-#pragma unroll
-                for (int m = 0; m < 20; ++m) vb[m] = v[m];
-            }
-            {
-                float bx = 0.f;
-#pragma unroll
-                for (int i = -R; i <= R; ++i) bx = bx + vb[8 + i];
-                tb[0] = bx;
-#pragma unroll
-                for (int q = 1; q < 4; ++q) {
-                    bx = (bx + vb[8 + q + R]) - vb[8 + q - 1 - R];
-                    tb[q] = bx;
-                }
-            }
-            const bool row_in = (vr >= 0) && (vr < H);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float gq = row_in ? cdiv_t(tg[q], kp.gn, kp.rgn, FAST) : 0.f;
+                const float gq = cdiv_t(tg[q], kp.gn, kp.rgn, FAST);
                 vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
                 rg[s][q] = gq;
                 rb[s][q] = tb[q];
@@ -379,32 +395,25 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
                         gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
                     const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
                                       cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
-                    out[q] = (res < 0.f) ? 0.f : res;
+                    out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
                 }
-                if (cb + 3 < W) {
-                    *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cb) =
+                if (outl) {
+                    *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cc) =
                         make_float4(out[0], out[1], out[2], out[3]);
                 }
-                if (kp.hist && cb + 3 < W) {
-                    // neighbouring pixels usually share a bin: merge runs of
-                    // equal keys before touching the LDS histogram
-                    uint32_t pk = 0xffffffffu, pc = 0;
+                if (kp.hist && outl) {
+                    // key (bits >> 19) + 1 for positive values (finite and
+                    // >= +0, so no clamp is needed); zero pixels are counted
+                    // in a word of the lane's own, away from the shared bins
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         const uint32_t bits = __float_as_uint(out[q]);
-                        if (bits == 0) {
-                            ++zc;
-                        } else {
-                            const uint32_t key = pct_key(bits);
-                            if (key != pk) {
-                                if (pc) atomicAdd(&s_hist[pk], pc);
-                                pk = key;
-                                pc = 0;
-                            }
-                            ++pc;
-                        }
+                        const uint32_t k = (bits >> 19) + 1u;
+                        const bool nz = bits != 0u;
+                        const uint32_t word = nz ? (k >> 1) : (uint32_t)(HIST_BINS / 2 + t);
+                        const uint32_t val = (nz && (k & 1u)) ? 0x10000u : 1u;
+                        atomicAdd(&s_hist[word], val);
                     }
-                    if (pc) atomicAdd(&s_hist[pk], pc);
                 }
             }
             }  // vr < vr1
@@ -414,11 +423,12 @@ __global__ __launch_bounds__(64) void k_bandpass(Batch b, K1Params kp) {
     if (kp.hist) {
         __syncthreads();
         uint32_t* gh = b.hist1 + (size_t)f * HIST_BINS;
-        for (int i = t; i < HIST_BINS; i += 64) {
+        for (int i = t; i < HIST_BINS / 2; i += 64) {
             const uint32_t c = s_hist[i];
-            if (c) atomicAdd(&gh[i], c);
+            if (c & 0xffffu) atomicAdd(&gh[2 * i], c & 0xffffu);
+            if (c >> 16) atomicAdd(&gh[2 * i + 1], c >> 16);
         }
-        const uint32_t z = wave_sum_u(zc);
+        const uint32_t z = wave_sum_u(s_hist[HIST_BINS / 2 + t]);
         if (t == 0 && z) atomicAdd(&gh[0], z);
     }
 }
@@ -520,6 +530,8 @@ hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, in
     kp.gn = g.gn; kp.rgn = g.rgn; kp.kk = g.kk; kp.rkk = g.rkk;
     kp.seg = seg_rows;
     kp.hist = b.hist1 != nullptr && g.has_pct;
+    // the LDS histogram keeps 16-bit bins
+    if (kp.hist && (int64_t)K1_STRIP * seg_rows >= 65536) return hipErrorInvalidValue;
     const bool fast = g.fastdiv_gn && g.fastdiv_kk;
     if (b.in_type == IN_U16) return launch_bp_r<uint16_t>(g.R, b, kp, fast, s);
     return launch_bp_r<uint8_t>(g.R, b, kp, fast, s);
@@ -602,6 +614,10 @@ struct DetParams {
     int band;
     int diw;            // disk inside window (zero-valued maxima cannot yield rows)
     int has_pct;
+    int walk_ri;        // disk radius of the walk run inside detect (0: none)
+    uint32_t maxit;
+    float minmass;
+    uint64_t slot_cap;
 };
 
 // y[i] = max(x[i .. i+K-1]) for i in [0, N) (van Herk / Gil-Werman): the
@@ -632,12 +648,107 @@ __device__ __forceinline__ void sliding_max(const float (&x)[N + K - 1], float (
     for (int i = 0; i < N; ++i) y[i] = fmaxf(S[i], Pm[i + K - 1]);
 }
 
+constexpr int DET_CAND_CAP = 1024;   // candidate pixels kept in LDS per tile
+
+// Centroid walk of the tile's candidates (centers.wgsl:23-109 and the walk
+// of walk_cols.wgsl, as in k_refine): candidate k of the tile (raster
+// order) walks in lane k % 256 of the workgroup, so a typical tile (a few
+// dozen candidates) occupies part of one wave.  Disks inside the staged area
+// are summed from LDS with the taps in the shader's order, others from P.
+template <int RI, int LR, int LC, int LS>
+__device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
+                                          const float* __restrict__ L,
+                                          const uint16_t* __restrict__ s_cand,
+                                          const GLOBAL uint16_t* __restrict__ slots,
+                                          uint32_t total, uint32_t item,
+                                          const GLOBAL float* __restrict__ P, int H, int W,
+                                          int x0, int y0, int a, int tid) {
+    constexpr int BW = 2 * RI + 1;
+    const float sthr = 0.6f;
+    const int band = dp.band;
+    const size_t slot0 = (size_t)item * TILE_PX;
+    const uint64_t cap = dp.slot_cap;
+    float* rec = b.rec;
+    for (uint32_t k = (uint32_t)tid; k < total; k += 256) {
+        const uint32_t l = k < (uint32_t)DET_CAND_CAP ? s_cand[k] : slots[k];
+        const int lr0 = (int)(l / TILE_W), lc0 = (int)(l % TILE_W);
+        int u = y0 + lr0, v = x0 + lc0;
+        float m = 0.f, c0 = 0.f, c1 = 0.f;
+        for (uint32_t it = 0; it < dp.maxit; ++it) {
+            if (u - band < 0 || u + band >= H || v - band < 0 || v + band >= W) {
+                m = 0.f; c0 = 0.f; c1 = 0.f;
+            } else {
+                // top-left corner of the disk's box in the staged area
+                const int br = u - y0 + a - RI, bc = v - x0 + a - RI;
+                float s0 = 0.f, s1 = 0.f, ms = 0.f;
+                if (br >= 0 && br + BW <= LR && bc >= 0 && bc + BW <= LC) {
+                    const float* box = L + br * LS + bc;
+#pragma unroll
+                    for (int di = -RI; di <= RI; ++di)
+#pragma unroll
+                        for (int dj = -RI; dj <= RI; ++dj) {
+                            if (di * di + dj * dj > RI * RI) continue;
+                            const float d = box[(di + RI) * LS + (dj + RI)];
+                            s0 = __builtin_fmaf(d, (float)di, s0);
+                            s1 = __builtin_fmaf(d, (float)dj, s1);
+                            ms = ms + d;
+                        }
+                } else {
+#pragma unroll
+                    for (int di = -RI; di <= RI; ++di) {
+                        const GLOBAL float* row = P + (size_t)(u + di) * W + v;
+#pragma unroll
+                        for (int dj = -RI; dj <= RI; ++dj) {
+                            if (di * di + dj * dj > RI * RI) continue;
+                            const float d = row[dj];
+                            s0 = __builtin_fmaf(d, (float)di, s0);
+                            s1 = __builtin_fmaf(d, (float)dj, s1);
+                            ms = ms + d;
+                        }
+                    }
+                }
+                m = ms;
+                c0 = s0 / ms;
+                c1 = s1 / ms;
+            }
+            bool changed = false;
+            if (c0 > sthr && u < H - 1) { u += 1; changed = true; }
+            else if (c0 < -sthr && u > 0) { u -= 1; changed = true; }
+            if (c1 > sthr && v < W - 1) { v += 1; changed = true; }
+            else if (c1 < -sthr && v > 0) { v -= 1; changed = true; }
+            if (!changed) break;
+        }
+        const bool valid = (m > dp.minmass) && (m != 0.f);
+        rec[REC_MASS * cap + slot0 + k] = valid ? m : 0.f;
+        rec[REC_Y * cap + slot0 + k] = (float)u + c0;
+        rec[REC_X * cap + slot0 + k] = (float)v + c1;
+        if (dp.has_pct) rec[REC_CENTER * cap + slot0 + k] = L[(lr0 + a) * LS + (lc0 + a)];
+    }
+}
+
+// Bit-pattern range [base, base + width) of percentile histogram bin k
+// (width 0 for bin 0 and for "no bin"): a value is listed when
+// bits - base < width, unsigned.
+__device__ __forceinline__ void pct_bin_range(int32_t k, uint32_t& base, uint32_t& width) {
+    if (k <= 0) { base = 0; width = 0; return; }
+    base = (uint32_t)(k - 1) << 19;
+    width = (k >= HIST_BINS - 1) ? 0x80000000u - base : (1u << 19);
+}
+
 // One workgroup per 64x64 tile.  The tile and its window halo are staged in
 // LDS (all loads issued before the first store); the row maxima overwrite the
 // staged tile in place, so one tile needs only (64+KD-1) x (64+KD+3) floats of
 // LDS and about six workgroups share a CU, overlapping each other's load
 // latency.
-template <int KD>
+//
+// RI > 0: the centroid walk of every candidate (refine) runs here too, on
+// the staged tile: after the candidate test the tile's columns are staged
+// again from P (the row maxima overwrote them) and each candidate walks in
+// one lane, reading its disk from LDS (from P when the walk leaves the
+// staged area).  Every candidate walks; the percentile test on its start
+// value is applied in dedup, which reads that value from the REC_CENTER
+// plane.
+template <int KD, int RI>
 __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int N = 16;
     constexpr int LR = TILE_H + KD - 1;
@@ -646,10 +757,12 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int NIN = N + KD - 1;
     constexpr int NV4 = (NIN + 3) / 4;
     constexpr int NRUN = (LR * 4 + 255) / 256;
+    static_assert(LC - 64 <= 64, "halo wider than one extra column pass");
     __shared__ __attribute__((aligned(16))) float L[LR * LS];
-    __shared__ uint64_t s_mask[TILE_H];
+    __shared__ uint32_t s_rowcnt[TILE_H];
     __shared__ uint32_t s_off[TILE_H];
-    __shared__ uint32_t s_lcnt;
+    __shared__ uint32_t s_lcnt, s_total;
+    __shared__ uint16_t s_cand[RI > 0 ? DET_CAND_CAP : 1];
 
     const int tid = threadIdx.x;
     const int a = dp.a;
@@ -663,37 +776,53 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     const GLOBAL float* P = (const GLOBAL float*)fr.P;
     if (tid == 0) s_lcnt = 0;
 
-    {
-        // wave w stages rows w, w+4, ...: lane = column (plus a second pass
-        // over the KD-1 halo columns); all loads are issued before any store
+    // Staging: wave w fills rows w, w+4, ...: lane = column (plus a second
+    // pass over the KD-1 halo columns).  Indices are clamped into the frame
+    // so every load is in bounds; out-of-frame cells become -inf.  All loads
+    // are issued before any store.
+    auto stage_tile = [&](bool halo) {
         constexpr int RPW = (LR + 3) / 4;
-        const int lane = tid & 63, wv = tid >> 6;
+        const int lane = tid & 63;
+        const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
         const int v1 = x0 - a + lane, v2 = v1 + 64;
         const bool c1 = v1 >= 0 && v1 < W;
-        const bool c2 = (lane < LC - 64) && v2 >= 0 && v2 < W;
+        const bool h2 = halo && lane < LC - 64;
+        const bool c2 = h2 && v2 >= 0 && v2 < W;
+        // byte offsets within a row (32-bit, so loads take a scalar base
+        // plus a vector offset)
+        const uint32_t o1 = 4u * (uint32_t)min(max(v1, 0), W - 1);
+        const uint32_t o2 = 4u * (uint32_t)min(max(v2, 0), W - 1);
         float f1[RPW], f2[RPW];
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
             const int r = wv + 4 * q;
             const int u = y0 - a + r;
-            const bool ur = r < LR && u >= 0 && u < H;
-            const GLOBAL float* prow = P + (size_t)(ur ? u : 0) * W;
-            f1[q] = (ur && c1) ? prow[v1] : __uint_as_float(NEG_INF_BITS);
-            f2[q] = (ur && c2) ? prow[v2] : __uint_as_float(NEG_INF_BITS);
+            const GLOBAL char* prow =
+                (const GLOBAL char*)(P + (size_t)min(max(u, 0), H - 1) * (size_t)W);
+            f1[q] = 0.f;
+            f2[q] = 0.f;
+            if (r < LR) {  // wave-uniform
+                f1[q] = *(const GLOBAL float*)(prow + o1);
+                if (h2) f2[q] = *(const GLOBAL float*)(prow + o2);
+            }
         }
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
             const int r = wv + 4 * q;
+            const int u = y0 - a + r;
             if (r < LR) {
-                L[r * LS + lane] = f1[q];
-                if (lane < LC - 64) L[r * LS + 64 + lane] = f2[q];
+                const bool ur = u >= 0 && u < H;  // wave-uniform
+                L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
+                if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
             }
         }
-    }
+    };
+    stage_tile(true);
     __syncthreads();
 
     // centre values of this thread's 16 outputs, read before the overwrite
-    const int c = tid & 63, kw = tid >> 6;
+    const int c = tid & 63;
+    const int kw = __builtin_amdgcn_readfirstlane(tid >> 6);
     float center[N];
 #pragma unroll
     for (int m = 0; m < N; ++m) center[m] = L[(kw * N + m + a) * LS + (c + a)];
@@ -743,57 +872,83 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         sliding_max<N, KD>(x, cm);
     }
 
-    int32_t bin_lo = -1, bin_hi = -1;
-    if (dp.has_pct) {
-        bin_lo = b.selstate[f * 4 + 0];
-        bin_hi = b.selstate[f * 4 + 2];
-    }
-    GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
-    uint32_t cbits = 0;
+    // candidates: the ballots of the wave's 16 rows stay in scalar registers
+    const bool colok = (v < W) && (v - dp.band >= 0) && (v + dp.band < W);
+    uint64_t bal[N];
+    uint32_t mycnt = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        const int r = kw * N + m;
-        const int u = y0 + r;
-        const bool inb = (u < H) && (v < W);
-        const bool interior = (u - dp.band >= 0) && (u + dp.band < H) &&
-                              (v - dp.band >= 0) && (v + dp.band < W);
-        const bool cand = inb && interior && (center[m] >= cm[m]) &&
+        const int u = y0 + kw * N + m;
+        const bool rowok = (u < H) && (u - dp.band >= 0) && (u + dp.band < H);  // uniform
+        const bool cand = colok && rowok && (center[m] >= cm[m]) &&
                           !(dp.diw && center[m] == 0.f);
-        const uint64_t msk = __ballot(cand);
-        if (c == 0) s_mask[r] = msk;
-        cbits |= (cand ? 1u : 0u) << m;
-        if (dp.has_pct) {
+        bal[m] = __ballot(cand);
+        if (c == m) mycnt = (uint32_t)__popcll(bal[m]);
+    }
+    if (c < N) s_rowcnt[kw * N + c] = mycnt;
+
+    if (dp.has_pct) {
+        // values in the two histogram bins holding the order statistics go to
+        // the tile's list (any order); out-of-frame centres are -inf and
+        // fall outside every range
+        uint32_t lb, lw, hb, hw;
+        pct_bin_range(b.selstate[f * 4 + 0], lb, lw);
+        pct_bin_range(b.selstate[f * 4 + 2], hb, hw);
+        uint32_t mbits = 0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
             const uint32_t bits = __float_as_uint(center[m]);
-            const int key = (int)pct_key(bits);
-            const bool match = inb && key != 0 && (key == bin_lo || key == bin_hi);
-            const uint64_t mm = __ballot(match);
-            if (mm) {
-                uint32_t basei = 0;
-                if (c == 0) basei = atomicAdd(&s_lcnt, (uint32_t)__popcll(mm));
-                basei = __shfl(basei, 0, 64);
-                if (match) list[basei + __popcll(mm & lanemask_lt(c))] = bits;
+            const bool match = (bits - lb < lw) || (bits - hb < hw);
+            mbits |= (match ? 1u : 0u) << m;
+        }
+        if (__ballot(mbits != 0u)) {  // wave-uniform
+            const uint32_t n = (uint32_t)__popc(mbits);
+            const uint32_t inc = wave_incl_scan(n, c);
+            const uint32_t tot = __shfl(inc, 63, 64);
+            uint32_t base = 0;
+            if (c == 0) base = atomicAdd(&s_lcnt, tot);
+            base = __builtin_amdgcn_readfirstlane(base);
+            uint32_t j = base + inc - n;
+            GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                if (mbits & (1u << m)) list[j++] = __float_as_uint(center[m]);
             }
         }
     }
     __syncthreads();
     if (tid < 64) {
-        const uint32_t cnt = (uint32_t)__popcll(s_mask[tid]);
+        const uint32_t cnt = s_rowcnt[tid];
         const uint32_t inc = wave_incl_scan(cnt, tid);
         s_off[tid] = inc - cnt;
         if (tid == 63) {
             b.tile_count[item] = inc;
+            s_total = inc;
             if (dp.has_pct) b.tile_lcount[item] = s_lcnt;
         }
     }
     __syncthreads();
     GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
+    const uint64_t below = lanemask_lt(c);
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        if (cbits & (1u << m)) {
-            const int r = kw * N + m;
-            const uint32_t rank = s_off[r] + (uint32_t)__popcll(s_mask[r] & lanemask_lt(c));
+        const int r = kw * N + m;
+        if ((bal[m] >> c) & 1ull) {
+            const uint32_t rank = s_off[r] + (uint32_t)__popcll(bal[m] & below);
             slots[rank] = (uint16_t)(r * TILE_W + c);
+            if constexpr (RI > 0) {
+                if (rank < (uint32_t)DET_CAND_CAP) s_cand[rank] = (uint16_t)(r * TILE_W + c);
+            }
         }
+    }
+
+    if constexpr (RI > 0) {
+        const uint32_t total = s_total;
+        if (total == 0) return;  // workgroup-uniform
+        __syncthreads();         // column pass done with L; slots visible
+        stage_tile(false);       // columns 0..63 again hold P
+        __syncthreads();
+        walk_tile<RI, LR, LC, LS>(b, dp, L, s_cand, slots, total, item, P, H, W, x0, y0, a, tid);
     }
 }
 
@@ -892,8 +1047,33 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
 
 template <int KD>
 static hipError_t launch_det_t(const Batch& b, const DetParams& dp, hipStream_t s) {
-    hipLaunchKernelGGL(k_detect<KD>, dim3(b.tiles), dim3(256), 0, s, b, dp);
+    switch (dp.walk_ri) {
+        case 0: hipLaunchKernelGGL((k_detect<KD, 0>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 1: hipLaunchKernelGGL((k_detect<KD, 1>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 2: hipLaunchKernelGGL((k_detect<KD, 2>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 3: hipLaunchKernelGGL((k_detect<KD, 3>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 4: hipLaunchKernelGGL((k_detect<KD, 4>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 5: hipLaunchKernelGGL((k_detect<KD, 5>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 6: hipLaunchKernelGGL((k_detect<KD, 6>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 7: hipLaunchKernelGGL((k_detect<KD, 7>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+static bool detect_specialised(int kd) {
+    switch (kd) {
+        case 4: case 5: case 7: case 8: case 9: case 11: case 12: case 14: case 15: case 16:
+        case 18: case 19: case 21:
+            return true;
+        default:
+            return false;
+    }
+}
+
+bool walk_in_detect_supported(const Geom& g) {
+    return detect_specialised(g.kd) && g.rint >= 1 && g.rint <= 7 && g.max_iterations >= 1 &&
+           !g.characterize && !g.cpu_processed;
 }
 
 hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
@@ -904,6 +1084,10 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
     dp.band = g.band;
     dp.diw = g.disk_in_window;
     dp.has_pct = g.has_pct;
+    dp.walk_ri = g.walk_in_detect ? g.rint : 0;
+    dp.maxit = g.max_iterations;
+    dp.minmass = g.minmass;
+    dp.slot_cap = b.slot_cap;
     switch (g.kd) {
         case 4: return launch_det_t<4>(b, dp, s);
         case 5: return launch_det_t<5>(b, dp, s);
@@ -1439,6 +1623,7 @@ __global__ __launch_bounds__(256) void k_refine_staged(Batch b, const Geom* __re
 }
 
 hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
+    if (g.walk_in_detect) return hipSuccess;  // done by detect
     const dim3 grid((b.tiles + RT_TILES - 1) / RT_TILES);
     if (g.max_iterations >= 1) {
         switch (g.rint) {
@@ -1490,6 +1675,14 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     const int tx = (int)(local % fr.tx), ty = (int)(local / fr.tx);
     const uint32_t cnt = b.tile_count[item];
     const size_t slot0 = (size_t)item * TILE_PX;
+    // when the walk ran in detect, every candidate walked and the percentile
+    // test on its start value (refine's `P > thr`) is applied here
+    const bool use_thr = g->walk_in_detect && g->has_pct;
+    const double thr = use_thr ? b.thr[f] : 0.0;
+    const GLOBAL float* rcen = (const GLOBAL float*)(b.rec + REC_CENTER * cap);
+    auto ok = [&](float mass, size_t s) {
+        return mass != 0.f && (!use_thr || (double)rcen[s] > thr);
+    };
     const int y_lo = max(0, ty - rty), y_hi = min((int)fr.ty - 1, ty + rty);
     const int x_lo = max(0, tx - rtx), x_hi = min((int)fr.tx - 1, tx + rtx);
     const int nxr = x_hi - x_lo + 1;
@@ -1498,7 +1691,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     // bounding box of the tile's own valid positions
     float bymin = INFINITY, bymax = -INFINITY, bxmin = INFINITY, bxmax = -INFINITY;
     for (uint32_t i = lane; i < cnt; i += 64) {
-        if (rmass[slot0 + i] != 0.f) {
+        if (ok(rmass[slot0 + i], slot0 + i)) {
             const float y = ry[slot0 + i], x = rx[slot0 + i];
             bymin = fminf(bymin, y); bymax = fmaxf(bymax, y);
             bxmin = fminf(bxmin, x); bxmax = fmaxf(bxmax, x);
@@ -1536,7 +1729,8 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                     m = rmass[s2 + j];
                     y = ry[s2 + j];
                     x = rx[s2 + j];
-                    take = m != 0.f && y >= bymin && y <= bymax && x >= bxmin && x <= bxmax;
+                    take = ok(m, s2 + j) && y >= bymin && y <= bymax && x >= bxmin &&
+                           x <= bxmax;
                 }
                 const uint64_t tm = __ballot(take);
                 const uint32_t nt = (uint32_t)__popcll(tm);
@@ -1569,7 +1763,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
             const uint32_t l = slots[slot0 + i];
             idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
         }
-        const bool valid = own && (m != 0.f);
+        const bool valid = own && ok(m, slot0 + i);
         bool keep = valid;
         if (staged) {
             // one own candidate at a time, staged records across the lanes
@@ -1604,7 +1798,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                     const size_t s2 = (size_t)t2 * TILE_PX;
                     for (uint32_t j = 0; j < c2; ++j) {
                         const float nm = rmass[s2 + j];
-                        if (nm == 0.f) continue;
+                        if (!ok(nm, s2 + j)) continue;
                         const float dy = ry[s2 + j] - y, dx = rx[s2 + j] - x;
                         float d = dy * dy;
                         d = d + dx * dx;
