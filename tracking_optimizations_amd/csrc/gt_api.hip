@@ -157,7 +157,8 @@ gt_status check_params(const gt_params* p, uint32_t h, uint32_t w) {
                         "frame %ux%u is smaller than smoothing_size/2 = %u (single mirror "
                         "reflection of the boxcar would leave the frame)",
                         h, w, R);
-        if ((uint64_t)h * w >= (1ull << 31)) return fail(GT_E_INVALID, "frame too large");
+        // kernels address a frame's float32 plane with 32-bit byte offsets
+        if ((uint64_t)h * w >= (1ull << 30)) return fail(GT_E_INVALID, "frame too large");
     }
     return GT_OK;
 }

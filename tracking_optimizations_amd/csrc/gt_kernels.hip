@@ -780,37 +780,42 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     // pass over the KD-1 halo columns).  Indices are clamped into the frame
     // so every load is in bounds; out-of-frame cells become -inf.  All loads
     // are issued before any store.
+    // Tiles whose staged area lies inside the frame (most of them) skip the
+    // -inf selects.  Byte offsets are 32-bit (frames hold < 2^30 pixels), so
+    // the loads take the frame base as a scalar address.
+    const bool inner = (x0 - a >= 0) && (x0 - a + LC <= W) && (y0 - a >= 0) && (y0 - a + LR <= H);
+    const GLOBAL char* Pb = (const GLOBAL char*)P;
     auto stage_tile = [&](bool halo) {
         constexpr int RPW = (LR + 3) / 4;
         const int lane = tid & 63;
         const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
         const int v1 = x0 - a + lane, v2 = v1 + 64;
-        const bool c1 = v1 >= 0 && v1 < W;
         const bool h2 = halo && lane < LC - 64;
-        const bool c2 = h2 && v2 >= 0 && v2 < W;
-        // byte offsets within a row (32-bit, so loads take a scalar base
-        // plus a vector offset)
         const uint32_t o1 = 4u * (uint32_t)min(max(v1, 0), W - 1);
         const uint32_t o2 = 4u * (uint32_t)min(max(v2, 0), W - 1);
         float f1[RPW], f2[RPW];
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
-            const int r = wv + 4 * q;
+            const int r = min(wv + 4 * q, LR - 1);  // repeated last row: same values
             const int u = y0 - a + r;
-            const GLOBAL char* prow =
-                (const GLOBAL char*)(P + (size_t)min(max(u, 0), H - 1) * (size_t)W);
-            f1[q] = 0.f;
-            f2[q] = 0.f;
-            if (r < LR) {  // wave-uniform
-                f1[q] = *(const GLOBAL float*)(prow + o1);
-                if (h2) f2[q] = *(const GLOBAL float*)(prow + o2);
-            }
+            const uint32_t rb = 4u * (uint32_t)(min(max(u, 0), H - 1) * W);
+            f1[q] = *(const GLOBAL float*)(Pb + (rb + o1));
+            if (h2) f2[q] = *(const GLOBAL float*)(Pb + (rb + o2));
         }
+        if (inner) {  // workgroup-uniform
 #pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-            const int r = wv + 4 * q;
-            const int u = y0 - a + r;
-            if (r < LR) {
+            for (int q = 0; q < RPW; ++q) {
+                const int r = min(wv + 4 * q, LR - 1);
+                L[r * LS + lane] = f1[q];
+                if (h2) L[r * LS + 64 + lane] = f2[q];
+            }
+        } else {
+            const bool c1 = v1 >= 0 && v1 < W;
+            const bool c2 = v2 >= 0 && v2 < W;
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                const int r = min(wv + 4 * q, LR - 1);
+                const int u = y0 - a + r;
                 const bool ur = u >= 0 && u < H;  // wave-uniform
                 L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
                 if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
@@ -891,15 +896,19 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         // values in the two histogram bins holding the order statistics go to
         // the tile's list (any order); out-of-frame centres are -inf and
         // fall outside every range
+        // The two order statistics have adjacent ranks, so no value lies
+        // strictly between their bins: one range from the start of the lower
+        // bin to the end of the upper one holds exactly the two bins.
         uint32_t lb, lw, hb, hw;
         pct_bin_range(b.selstate[f * 4 + 0], lb, lw);
         pct_bin_range(b.selstate[f * 4 + 2], hb, hw);
+        const uint32_t base = lw ? lb : hb;
+        const uint32_t width = hw ? (hb + hw - base) : lw;
         uint32_t mbits = 0;
 #pragma unroll
-        for (int m = 0; m < N; ++m) {
+        for (int m = N - 1; m >= 0; --m) {
             const uint32_t bits = __float_as_uint(center[m]);
-            const bool match = (bits - lb < lw) || (bits - hb < hw);
-            mbits |= (match ? 1u : 0u) << m;
+            mbits = 2u * mbits + ((bits - base < width) ? 1u : 0u);
         }
         if (__ballot(mbits != 0u)) {  // wave-uniform
             const uint32_t n = (uint32_t)__popc(mbits);
@@ -929,12 +938,13 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     }
     __syncthreads();
     GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
-    const uint64_t below = lanemask_lt(c);
 #pragma unroll
     for (int m = 0; m < N; ++m) {
         const int r = kw * N + m;
         if ((bal[m] >> c) & 1ull) {
-            const uint32_t rank = s_off[r] + (uint32_t)__popcll(bal[m] & below);
+            const uint32_t rank =
+                s_off[r] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[m] >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal[m], 0u));
             slots[rank] = (uint16_t)(r * TILE_W + c);
             if constexpr (RI > 0) {
                 if (rank < (uint32_t)DET_CAND_CAP) s_cand[rank] = (uint16_t)(r * TILE_W + c);
