@@ -78,8 +78,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
-    ap.add_argument("--workload", choices=["movie", "uneven"], default="movie",
-                    help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames")
+    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint"], default="movie",
+                    help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames; "
+                         "adjoint: 256 x 512^2 localisation errors and Jacobians")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -92,6 +93,8 @@ def main() -> int:
     from tracking_optimizations_amd.gpu_tracking import make_params
     if args.workload == "uneven":
         return bench_uneven(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
     if args.no_percentile:
@@ -309,6 +312,70 @@ def bench_uneven(args) -> int:
                       "pixels_per_s": movie.pixels * args.steps / el,
                       "plain_frames_per_s": nf / pel, "batched_over_plain": fps * pel / nf}),
           flush=True)
+    movie.free()
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """Adjoint workload: 256 frames of 512x512 uint16 (300 spots, sigma 1.5,
+    peaks U(200, 3000), background 100 + Poisson), diameter 9, separation 9,
+    smoothing 9, percentile 64: forward rows plus per-particle Poisson errors
+    (and, second line of the log, with the Jacobian patches kept)."""
+    import ctypes
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import make_params
+    kw = dict(diameter=9, separation=9, smoothing_size=9, percentile=64, minmass=0.0)
+    params = make_params(**kw)
+    ctx = _lib.Context(0)
+    L = _lib.load()
+    nf = 256 if args.frames == 1000 else args.frames
+    spec = synth.adjoint_512(seed=4, n_frames=nf)
+    movie = synth.generate(ctx, spec)
+    ctx.synchronize()
+
+    def run(flags):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_adjoint(ctx.handle, ctypes.c_void_p(movie.ptr), 0, nf, 512, 512,
+                                       ctypes.byref(params), flags, ctypes.byref(out)))
+        a = _lib.Adjoint(out, nf, 512, 512)
+        return a.nrows
+
+    res = {}
+    for name, flags in (("errors", _lib.GT_FRAMES_ON_DEVICE),
+                        ("jacobian", _lib.GT_FRAMES_ON_DEVICE | _lib.GT_ADJ_JACOBIAN)):
+        for _ in range(args.warmup):
+            run(flags)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            nrows = run(flags)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        res[name] = (nf * args.steps / el, nrows, 1000 * el / args.steps)
+        log(f"adjoint ({name}): {res[name][0]:.0f} frames/s, {nrows} rows, "
+            f"{res[name][2]:.1f} ms/step")
+    # forward alone on the same frames, for the cost of the adjoint
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), nf, 512, 512,
+                                   ctypes.byref(params),
+                                   _lib.GT_FRAMES_ON_DEVICE | _lib.GT_RESULT_ON_DEVICE,
+                                   ctypes.byref(out)))
+        _lib.Result(out)
+    ctx.synchronize()
+    fwd = nf * args.steps / (time.perf_counter() - t0)
+    print(json.dumps({"metric": "frames/s, locate + adjoint (Poisson localisation errors), "
+                                "256 x 512^2 uint16",
+                      "value": res["errors"][0], "unit": "frames/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": res["errors"][2], "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic (seeded device generator), device-resident",
+                      "config": {"workload": "adjoint 256 x 512x512, 300 spots/frame",
+                                 "params": kw, "rows": res["errors"][1]},
+                      "with_jacobian_frames_per_s": res["jacobian"][0],
+                      "forward_only_frames_per_s": fwd}), flush=True)
     movie.free()
     return 0
 

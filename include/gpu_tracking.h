@@ -96,6 +96,7 @@ typedef struct gt_result gt_result;
 #define GT_FRAMES_ON_DEVICE  0x1u  /* frame pointers are HIP device pointers */
 #define GT_PLAIN             0x2u  /* per-frame pipeline: launch + readback per frame */
 #define GT_RESULT_ON_DEVICE  0x4u  /* leave rows in device memory (see gt_result_device_data) */
+#define GT_ADJ_JACOBIAN      0x8u  /* gt_locate_adjoint: keep the Jacobian patches */
 
 /* Per-call device timing of the last locate call on a context, measured with
  * HIP events on the context's stream. */
@@ -209,6 +210,44 @@ const float* gt_result_device_data(const gt_result* r);
 gt_status gt_result_columns(const gt_result* r, const char** names,
                             const char** dtypes, uint32_t cap, uint32_t* ncols);
 void gt_result_free(gt_result* r);
+
+/* --- adjoint: localisation error and gradients ------------------------- */
+
+/* Adjoint of the locate path (refine and bandpass) for the rows of one
+ * batched locate of n_frames frames of h x w (dtype 0 u16, 1 f32, 2 u8).
+ * No counterpart in the reference, which has no gradients; it extends
+ * execute_ndarray (execute_gpu.rs:496) for callers that propagate Poisson
+ * shot noise into per-particle localisation errors or differentiate through
+ * locate.  For every output row, with the walk's discrete decisions held
+ * fixed, it forms the Jacobian of (y, x, mass) with respect to the input
+ * pixels on an S x S patch (S = 2 * ((diameter-1)/2 + smoothing_size/2) + 1)
+ * centred on the particle's last evaluated disk, and the Poisson errors
+ * sigma = sqrt(sum_r J[r]^2 * max(I[r], 0)).  Flags: GT_FRAMES_ON_DEVICE,
+ * GT_ADJ_JACOBIAN (keep the patches, required by gt_adjoint_vjp).  The rows
+ * are those gt_locate_* returns for the same input. */
+typedef struct gt_adjoint gt_adjoint;
+gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
+                            uint32_t h, uint32_t w, const gt_params* p, uint32_t flags,
+                            gt_adjoint** out);
+uint64_t gt_adjoint_rows(const gt_adjoint* a);
+uint32_t gt_adjoint_ncols(const gt_adjoint* a);
+uint32_t gt_adjoint_patch(const gt_adjoint* a);
+/* Forward rows (as gt_result_data) and candidate pixels. */
+const float* gt_adjoint_data(const gt_adjoint* a);
+const int64_t* gt_adjoint_candidates(const gt_adjoint* a);
+/* n_rows x 3: sigma_y, sigma_x, sigma_mass. */
+const float* gt_adjoint_sigma(const gt_adjoint* a);
+/* n_rows x 2: frame row and column of each patch's corner. */
+const int32_t* gt_adjoint_origin(const gt_adjoint* a);
+/* Copies the patches, n_rows x 3 (y, x, mass) x S x S floats. */
+gt_status gt_adjoint_jacobian(const gt_adjoint* a, float* out, uint64_t capacity_floats);
+/* Vector-Jacobian product: grad_frames (n_frames x h x w) = sum over rows of
+ * grad_rows[row] . J[row] (grad_rows: n_rows x 3, d/dy, d/dx, d/dmass), added
+ * in row order (deterministic).  GT_FRAMES_ON_DEVICE: both are device
+ * pointers.  Needs GT_ADJ_JACOBIAN at gt_locate_adjoint. */
+gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad_rows,
+                         float* grad_frames, uint32_t flags);
+void gt_adjoint_free(gt_adjoint* a);
 
 /* --- synthetic movies (device side, for tests and benchmarks) ------------ */
 

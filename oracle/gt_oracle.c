@@ -168,3 +168,120 @@ int orc_dedup_bruteforce(const float* y, const float* x, const float* mass,
     }
     return 0;
 }
+
+/* ---- adjoint (no reference counterpart: checks the GPU adjoint) ---------- */
+
+/* Boxcar multiplicity of input index r in the window of q under the single
+ * mirror reflection (preprocess_rows.wgsl / preprocess_cols.wgsl boxcar). */
+static double box_count(int32_t q, int32_t r, int32_t R, int32_t n) {
+    double c = 0.0;
+    for (int32_t d = -R; d <= R; d++) {
+        int32_t t = q + d;
+        if (t < 0) t = -t - 1;
+        else if (t >= n) t = 2 * n - t - 1;
+        if (t == r) c += 1.0;
+    }
+    return c;
+}
+
+int orc_adjoint_particle(const float* frame, int32_t H, int32_t W, const orc_params* p,
+                         int32_t f64_forward, int64_t cand, double* jac, int32_t* S_out,
+                         int32_t* origin, double* yxm, int32_t* state) {
+    orc_geom g;
+    if (orc_geometry(p, &g) != 0) return -1;
+    const size_t npx = (size_t)H * W;
+    const int32_t R = g.R, r = g.rint, S = 2 * (r + R) + 1;
+    double* Pd = (double*)malloc(sizeof(double) * npx);
+    double* w = (double*)malloc(sizeof(double) * (size_t)(2 * R + 1));
+    if (!Pd || !w) { free(Pd); free(w); return -2; }
+    double gn, m, c0, c1;
+    int32_t st[4];
+    const int32_t u0 = (int32_t)(cand / W), v0 = (int32_t)(cand % W);
+    if (f64_forward) {
+        double* fd = (double*)malloc(sizeof(double) * npx);
+        if (!fd) { free(Pd); free(w); return -2; }
+        for (size_t i = 0; i < npx; i++) fd[i] = (double)frame[i];
+        if (bandpass_f64(fd, H, W, g.k, (double)p->noise_size, Pd) != 0) {
+            free(fd); free(Pd); free(w); return -1;
+        }
+        free(fd);
+        weights_f64(R, (double)p->noise_size, w, &gn);
+        walk_state_f64(Pd, H, W, &g, p->max_iterations, u0, v0, st, &m, &c0, &c1);
+    } else {
+        float* Pf = (float*)malloc(sizeof(float) * npx);
+        float* wf = (float*)malloc(sizeof(float) * (size_t)(2 * R + 1));
+        if (!Pf || !wf) { free(Pf); free(wf); free(Pd); free(w); return -2; }
+        if (bandpass_f32(frame, H, W, g.k, p->noise_size, Pf) != 0) {
+            free(Pf); free(wf); free(Pd); free(w); return -1;
+        }
+        float gnf, mf, c0f, c1f;
+        weights_f32(R, p->noise_size, wf, &gnf);
+        walk_state_f32(Pf, H, W, &g, p->max_iterations, u0, v0, st, &mf, &c0f, &c1f);
+        for (size_t i = 0; i < npx; i++) Pd[i] = (double)Pf[i];
+        for (int32_t i = 0; i < 2 * R + 1; i++) w[i] = (double)wf[i];
+        gn = (double)gnf; m = (double)mf; c0 = (double)c0f; c1 = (double)c1f;
+        free(Pf); free(wf);
+    }
+    const int32_t ue = st[0], ve = st[1];
+    if (state) for (int q = 0; q < 4; q++) state[q] = st[q];
+    if (yxm) { yxm[0] = (double)st[2] + c0; yxm[1] = (double)st[3] + c1; yxm[2] = m; }
+    const int32_t py0 = ue - r - R, px0 = ve - r - R;
+    if (origin) { origin[0] = py0; origin[1] = px0; }
+    if (S_out) *S_out = S;
+    const double gn2 = gn * gn, kk = (double)(g.k * g.k);
+    if (jac) {
+        memset(jac, 0, sizeof(double) * 3 * (size_t)S * S);
+        for (int32_t i = -r; i <= r; i++)
+            for (int32_t j = -r; j <= r; j++) {
+                if (i * i + j * j > r * r) continue;
+                const int32_t qy = ue + i, qx = ve + j;
+                if (!(Pd[(size_t)qy * W + qx] > 0.0)) continue;   /* clamp active */
+                const double a[3] = {((double)i - c0) / m, ((double)j - c1) / m, 1.0};
+                for (int32_t sy = 0; sy < S; sy++) {
+                    const int32_t ry = py0 + sy;
+                    if (ry < 0 || ry >= H) continue;
+                    const double wy = abs(ry - qy) <= R ? w[ry - qy + R] : 0.0;
+                    const double by = box_count(qy, ry, R, H);
+                    for (int32_t sx = 0; sx < S; sx++) {
+                        const int32_t rx = px0 + sx;
+                        if (rx < 0 || rx >= W) continue;
+                        const double wx = abs(rx - qx) <= R ? w[rx - qx + R] : 0.0;
+                        const double k = wy * wx / gn2 - by * box_count(qx, rx, R, W) / kk;
+                        for (int o = 0; o < 3; o++)
+                            jac[((size_t)o * S + sy) * S + sx] += a[o] * k;
+                    }
+                }
+            }
+    }
+    free(Pd); free(w);
+    return 0;
+}
+
+int orc_particle_f64(const double* frame, int32_t H, int32_t W, const orc_params* p,
+                     int64_t cand, double* yxm, int32_t* state, int32_t* n_active) {
+    orc_geom g;
+    if (orc_geometry(p, &g) != 0) return -1;
+    const size_t npx = (size_t)H * W;
+    double* Pd = (double*)malloc(sizeof(double) * npx);
+    if (!Pd) return -2;
+    if (bandpass_f64(frame, H, W, g.k, (double)p->noise_size, Pd) != 0) { free(Pd); return -1; }
+    double m, c0, c1;
+    int32_t st[4];
+    walk_state_f64(Pd, H, W, &g, p->max_iterations, (int32_t)(cand / W), (int32_t)(cand % W), st,
+                   &m, &c0, &c1);
+    yxm[0] = (double)st[2] + c0;
+    yxm[1] = (double)st[3] + c1;
+    yxm[2] = m;
+    if (state) for (int q = 0; q < 4; q++) state[q] = st[q];
+    if (n_active) {
+        int32_t na = 0;
+        for (int32_t i = -g.rint; i <= g.rint; i++)
+            for (int32_t j = -g.rint; j <= g.rint; j++) {
+                if (i * i + j * j > g.rint * g.rint) continue;
+                if (Pd[(size_t)(st[0] + i) * W + (st[1] + j)] > 0.0) na++;
+            }
+        *n_active = na;
+    }
+    free(Pd);
+    return 0;
+}

@@ -106,6 +106,22 @@ int orc_dedup_bruteforce(const float* y, const float* x, const float* mass,
                          const int64_t* idx, int64_t n, uint32_t separation,
                          uint8_t* keep);
 
+/* Adjoint of one particle started at candidate pixel `cand` (no reference
+ * counterpart; checks the GPU adjoint).  Forward state (bandpass, walk) in
+ * binary32 (f64_forward == 0) or binary64; the Jacobian itself in binary64:
+ * jac[3][S][S] = d(y, x, mass)/d(input pixel) on the patch whose corner is
+ * origin[0..1], S = 2*(rint + smoothing_size/2) + 1, walk decisions fixed.
+ * yxm: forward (y, x, mass); state: last evaluated disk centre (row, col)
+ * and final position (row, col). */
+int orc_adjoint_particle(const float* frame, int32_t H, int32_t W, const orc_params* p,
+                         int32_t f64_forward, int64_t cand, double* jac, int32_t* S_out,
+                         int32_t* origin, double* yxm, int32_t* state);
+
+/* Binary64 forward (bandpass + walk) of one particle from a binary64 frame,
+ * for finite differences; n_active counts the disk pixels with P > 0. */
+int orc_particle_f64(const double* frame, int32_t H, int32_t W, const orc_params* p,
+                     int64_t cand, double* yxm, int32_t* state, int32_t* n_active);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,12 @@ def lib():
                                            ctypes.c_int64, i64p]
         L.orc_dedup_bruteforce.argtypes = [fp, fp, fp, i64p, ctypes.c_int64,
                                            ctypes.c_uint32, u8p]
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.orc_adjoint_particle.argtypes = [fp, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(OrcParams), ctypes.c_int32,
+                                           ctypes.c_int64, dp, i32p, i32p, dp, i32p]
+        L.orc_particle_f64.argtypes = [dp, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(OrcParams), ctypes.c_int64, dp, i32p, i32p]
         _lib = L
     return _lib
 
@@ -245,3 +251,43 @@ def dedup_bruteforce(y, x, mass, idx, separation: int) -> np.ndarray:
                                _ptr(mass, ctypes.c_float), _ptr(idx, ctypes.c_int64),
                                len(y), separation, _ptr(keep, ctypes.c_uint8))
     return keep.astype(bool)
+
+
+def adjoint_particle(frame: np.ndarray, p: Params, cand: int, f64_forward: bool = True):
+    """Binary64 Jacobian of (y, x, mass) of the particle started at pixel
+    `cand` w.r.t. the input pixels, on its S x S patch.  Returns (jac
+    [3, S, S], origin (row, col), (y, x, mass), state (ue, ve, uf, vf))."""
+    f = np.ascontiguousarray(frame, np.float32)
+    H, W = f.shape
+    g = geometry(p)
+    S = 2 * (g.rint + g.R) + 1
+    jac = np.zeros((3, S, S), np.float64)
+    s_out = ctypes.c_int32()
+    origin = np.zeros(2, np.int32)
+    yxm = np.zeros(3, np.float64)
+    state = np.zeros(4, np.int32)
+    cp = p.c()
+    rc = lib().orc_adjoint_particle(_ptr(f, ctypes.c_float), H, W, ctypes.byref(cp),
+                                    int(f64_forward), int(cand), _ptr(jac, ctypes.c_double),
+                                    ctypes.byref(s_out), _ptr(origin, ctypes.c_int32),
+                                    _ptr(yxm, ctypes.c_double), _ptr(state, ctypes.c_int32))
+    if rc != 0:
+        raise RuntimeError(f"orc_adjoint_particle failed ({rc})")
+    return jac, (int(origin[0]), int(origin[1])), yxm, state
+
+
+def particle_f64(frame: np.ndarray, p: Params, cand: int):
+    """Binary64 forward of one particle from a float64 frame: ((y, x, mass),
+    state (ue, ve, uf, vf), number of disk pixels with P > 0)."""
+    f = np.ascontiguousarray(frame, np.float64)
+    H, W = f.shape
+    yxm = np.zeros(3, np.float64)
+    state = np.zeros(4, np.int32)
+    na = ctypes.c_int32()
+    cp = p.c()
+    rc = lib().orc_particle_f64(_ptr(f, ctypes.c_double), H, W, ctypes.byref(cp), int(cand),
+                                _ptr(yxm, ctypes.c_double), _ptr(state, ctypes.c_int32),
+                                ctypes.byref(na))
+    if rc != 0:
+        raise RuntimeError(f"orc_particle_f64 failed ({rc})")
+    return yxm, state, int(na.value)

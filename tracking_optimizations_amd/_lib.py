@@ -25,6 +25,7 @@ GT_E_INTERNAL = 6
 GT_FRAMES_ON_DEVICE = 0x1
 GT_PLAIN = 0x2
 GT_RESULT_ON_DEVICE = 0x4
+GT_ADJ_JACOBIAN = 0x8
 
 DTYPE_CODES = {np.dtype(np.uint16): 0, np.dtype(np.float32): 1, np.dtype(np.uint8): 2}
 
@@ -37,7 +38,10 @@ EXPORTS = [
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
-    "gt_memcpy_d2h", "gt_memcpy_d2d", "gt_synchronize", "gt_context_set_profiling", "gt_context_stage_times",
+    "gt_memcpy_d2h", "gt_memcpy_d2d", "gt_synchronize", "gt_locate_adjoint", "gt_adjoint_rows",
+    "gt_adjoint_ncols", "gt_adjoint_patch", "gt_adjoint_data", "gt_adjoint_candidates",
+    "gt_adjoint_sigma", "gt_adjoint_origin", "gt_adjoint_jacobian", "gt_adjoint_vjp",
+    "gt_adjoint_free", "gt_context_set_profiling", "gt_context_stage_times",
 ]
 
 STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
@@ -177,6 +181,26 @@ def load() -> ctypes.CDLL:
         L.gt_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_memcpy_d2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_synchronize.argtypes = [vp]
+        L.gt_locate_adjoint.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                        ctypes.c_uint32, P(GtParams), ctypes.c_uint32, P(vp)]
+        L.gt_adjoint_rows.argtypes = [vp]
+        L.gt_adjoint_rows.restype = ctypes.c_uint64
+        L.gt_adjoint_ncols.argtypes = [vp]
+        L.gt_adjoint_ncols.restype = ctypes.c_uint32
+        L.gt_adjoint_patch.argtypes = [vp]
+        L.gt_adjoint_patch.restype = ctypes.c_uint32
+        L.gt_adjoint_data.argtypes = [vp]
+        L.gt_adjoint_data.restype = P(ctypes.c_float)
+        L.gt_adjoint_candidates.argtypes = [vp]
+        L.gt_adjoint_candidates.restype = P(ctypes.c_int64)
+        L.gt_adjoint_sigma.argtypes = [vp]
+        L.gt_adjoint_sigma.restype = P(ctypes.c_float)
+        L.gt_adjoint_origin.argtypes = [vp]
+        L.gt_adjoint_origin.restype = P(ctypes.c_int32)
+        L.gt_adjoint_jacobian.argtypes = [vp, vp, ctypes.c_uint64]
+        L.gt_adjoint_vjp.argtypes = [vp, vp, vp, vp, ctypes.c_uint32]
+        L.gt_adjoint_free.argtypes = [vp]
+        L.gt_adjoint_free.restype = None
         L.gt_context_set_profiling.argtypes = [vp, ctypes.c_int]
         L.gt_context_stage_times.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64)]
         _lib = L
@@ -310,3 +334,70 @@ class Result:
 
     def device_ptr(self) -> int:
         return int(load().gt_result_device_data(self._h) or 0)
+
+
+class Adjoint:
+    """Owns a gt_adjoint: forward rows, Poisson localisation errors and
+    (optionally) the per-row Jacobian patches of (y, x, mass)."""
+
+    def __init__(self, handle: ctypes.c_void_p, nframes: int, h: int, w: int):
+        self._h = handle
+        self.nframes, self.h, self.w = nframes, h, w
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().gt_adjoint_free(self._h)
+            self._h = None
+
+    @property
+    def nrows(self) -> int:
+        return int(load().gt_adjoint_rows(self._h))
+
+    @property
+    def ncols(self) -> int:
+        return int(load().gt_adjoint_ncols(self._h))
+
+    @property
+    def patch(self) -> int:
+        return int(load().gt_adjoint_patch(self._h))
+
+    def _arr(self, fn, shape, dtype):
+        if self.nrows == 0:
+            return np.zeros(shape, dtype)
+        return np.ctypeslib.as_array(fn(self._h), shape=shape).copy()
+
+    def rows(self) -> np.ndarray:
+        return self._arr(load().gt_adjoint_data, (self.nrows, self.ncols), np.float32)
+
+    def candidates(self) -> np.ndarray:
+        return self._arr(load().gt_adjoint_candidates, (self.nrows,), np.int64)
+
+    def sigma(self) -> np.ndarray:
+        """n_rows x 3: sigma_y, sigma_x, sigma_mass."""
+        return self._arr(load().gt_adjoint_sigma, (self.nrows, 3), np.float32)
+
+    def origin(self) -> np.ndarray:
+        """n_rows x 2: (row, column) of each Jacobian patch's corner."""
+        return self._arr(load().gt_adjoint_origin, (self.nrows, 2), np.int32)
+
+    def jacobian(self) -> np.ndarray:
+        """n_rows x 3 (y, x, mass) x S x S."""
+        S = self.patch
+        out = np.zeros((self.nrows, 3, S, S), np.float32)
+        if self.nrows:
+            check(load().gt_adjoint_jacobian(self._h, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def vjp(self, ctx: "Context", grad_rows, grad_frames=None, on_device: bool = False):
+        """grad_frames = sum over rows of grad_rows[row] . J[row]; with
+        on_device both arguments are device pointers (ints)."""
+        L = load()
+        if on_device:
+            check(L.gt_adjoint_vjp(ctx.handle, self._h, ctypes.c_void_p(grad_rows),
+                                   ctypes.c_void_p(grad_frames), GT_FRAMES_ON_DEVICE))
+            return grad_frames
+        g = np.ascontiguousarray(grad_rows, np.float32).reshape(self.nrows, 3)
+        out = np.zeros((self.nframes, self.h, self.w), np.float32)
+        check(L.gt_adjoint_vjp(ctx.handle, self._h, g.ctypes.data_as(ctypes.c_void_p),
+                               out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out

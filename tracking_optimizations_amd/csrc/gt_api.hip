@@ -292,6 +292,27 @@ struct gt_result {
     std::vector<std::string> names, dtypes;
 };
 
+// Adjoint of one locate call: the forward rows plus, per row, the Poisson
+// localisation errors, the Jacobian patch origin and (on request) the
+// Jacobian patches, which stay in device memory for gt_adjoint_vjp.
+struct gt_adjoint {
+    gt_result* fwd = nullptr;
+    int device = 0;
+    uint32_t S = 0;
+    uint64_t nframes = 0;
+    uint32_t h = 0, w = 0;     // frame shape (0 when the call had mixed shapes)
+    bool jacobian = false;
+    std::vector<float> sigma;
+    std::vector<int32_t> origin, rowpos;
+    DevBuf d_jac, d_origin, d_frame_rows;
+    ~gt_adjoint() {
+        delete fwd;
+        d_jac.release();
+        d_origin.release();
+        d_frame_rows.release();
+    }
+};
+
 namespace {
 
 struct FrameIn {
@@ -391,8 +412,21 @@ void collect_stage_times(gt_context* ctx) {
     ctx->ev_used.clear();
 }
 
+// Device buffers an adjoint request fills while the batches' P is resident.
+struct AdjReq {
+    bool jacobian = false;
+    DevBuf sigma, origin, rowpos, jac;
+    ~AdjReq() {
+        sigma.release();
+        origin.release();
+        rowpos.release();
+        jac.release();
+    }
+};
+
 gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
-                     const gt_params* p, uint32_t flags, gt_result** out) {
+                     const gt_params* p, uint32_t flags, gt_result** out,
+                     AdjReq* adj = nullptr) {
     const auto t_start = std::chrono::steady_clock::now();
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
     if (!out) return fail(GT_E_INVALID, "out is NULL");
@@ -411,6 +445,8 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     Geom g;
     GT_TRY(make_geom(p, integer_input, g));
     g.walk_in_detect = ctx->fuse_walk && walk_in_detect_supported(g) ? 1 : 0;
+    if (adj && !adjoint_supported(g))
+        return fail(GT_E_NOT_IMPLEMENTED, "adjoint needs (2*rint+1)^2 <= 1024 and smoothing_size/2 <= 16");
     const int seg = ctx->seg_rows;
     const size_t es = elem_size(in_type);
     // vector loads need W % 4 == 0 and frame bases aligned to 4 pixels
@@ -505,9 +541,11 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     }
     HIP_TRY(ctx->counters.ensure(256));
 
-    // rows capacity: one row per 16 pixels, grown on overflow
-    uint64_t rows_cap = std::max<uint64_t>(4096, total_px / 16);
-    if (ctx->rows.bytes >= rows_cap * 4 * g.ncols && ctx->rows_cand.bytes >= rows_cap * 8) {
+    // rows capacity: one row per 16 pixels (per 256 when Jacobian patches
+    // ride along), grown on overflow
+    uint64_t rows_cap = std::max<uint64_t>(4096, total_px / (adj && adj->jacobian ? 256 : 16));
+    if (!(adj && adj->jacobian) && ctx->rows.bytes >= rows_cap * 4 * g.ncols &&
+        ctx->rows_cand.bytes >= rows_cap * 8) {
         rows_cap = std::min<uint64_t>(ctx->rows.bytes / (4 * (uint64_t)g.ncols), ctx->rows_cand.bytes / 8);
     }
 
@@ -568,9 +606,23 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     const Geom* d_geom = ctx->geom.as<const Geom>();
 
     uint64_t launches = 0;
+    const int S = adjoint_patch(g);
     for (int attempt = 0; attempt < 2; ++attempt) {
         HIP_TRY(ctx->rows.ensure(rows_cap * 4 * (uint64_t)g.ncols));
         HIP_TRY(ctx->rows_cand.ensure(rows_cap * 8));
+        AdjOut ao{};
+        if (adj) {
+            HIP_TRY(adj->sigma.ensure(rows_cap * 12));
+            HIP_TRY(adj->origin.ensure(rows_cap * 8));
+            HIP_TRY(adj->rowpos.ensure(rows_cap * 4));
+            ao.sigma = adj->sigma.as<float>();
+            ao.origin = adj->origin.as<int32_t>();
+            ao.rowpos = adj->rowpos.as<int32_t>();
+            if (adj->jacobian) {
+                HIP_TRY(adj->jac.ensure(rows_cap * 12 * (uint64_t)S * S));
+                ao.jac = adj->jac.as<float>();
+            }
+        }
         uint64_t* counters = ctx->counters.as<uint64_t>();  // [0] cand_total, [1] row_carry
         res->data.clear();
         res->cand.clear();
@@ -673,6 +725,10 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             if (nstreams > 1) HIP_TRY(hipEventRecord(ctx->ev_scan[si], st));
             HIP_TRY(stage(ctx, st, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, st); }));
             launches += 5;
+            if (adj) {
+                HIP_TRY(launch_adjoint(b, d_geom, ao, (uint32_t)bp.b0, st));
+                ++launches;
+            }
 
             if (plain && result_on_device) {
                 // per-frame completion, as the reference's get_work waits per frame
@@ -968,6 +1024,131 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
         v.push_back({frames[i].data, frames[i].h, frames[i].w, frames[i].frame_index});
     return run_locate(ctx, v, in_type, p, flags, out);
 }
+
+gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
+                            uint32_t h, uint32_t w, const gt_params* p, uint32_t flags,
+                            gt_adjoint** out) {
+    if (!out) return fail(GT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (!frames && n_frames) return fail(GT_E_INVALID, "frames is NULL");
+    if (dtype < 0 || dtype > 2) return fail(GT_E_INVALID, "dtype must be 0 (u16), 1 (f32) or 2 (u8)");
+    if (flags & GT_PLAIN) return fail(GT_E_INVALID, "the adjoint runs on the batched path");
+    const int in_type = dtype == 0 ? IN_U16 : (dtype == 1 ? IN_F32 : IN_U8);
+    GT_TRY(check_params(p, h, w));
+    std::vector<FrameIn> v;
+    v.reserve(n_frames);
+    const size_t fb = (size_t)h * w * elem_size(in_type);
+    for (uint64_t i = 0; i < n_frames; ++i)
+        v.push_back({(const char*)frames + i * fb, h, w, (int64_t)i});
+    AdjReq req;
+    req.jacobian = (flags & GT_ADJ_JACOBIAN) != 0;
+    gt_result* res = nullptr;
+    GT_TRY(run_locate(ctx, v, in_type, p, flags & ~(uint32_t)(GT_RESULT_ON_DEVICE | GT_ADJ_JACOBIAN),
+                      &res, &req));
+    gt_adjoint* a = new gt_adjoint();
+    a->fwd = res;
+    a->device = ctx->device;
+    {
+        Geom g;
+        make_geom(p, in_type != IN_F32, g);
+        a->S = (uint32_t)adjoint_patch(g);
+    }
+    a->nframes = n_frames;
+    a->h = h;
+    a->w = w;
+    a->jacobian = req.jacobian;
+    const uint64_t n = res->rows;
+    a->sigma.resize(n * 3);
+    a->origin.resize(n * 2);
+    a->rowpos.resize(n);
+    auto cleanup = [&](hipError_t e) {
+        delete a;
+        return fail(GT_E_HIP, "adjoint readback: %s", hipGetErrorString(e));
+    };
+    hipError_t e = hipSuccess;
+    if (n) {
+        e = hipMemcpy(a->sigma.data(), req.sigma.p, n * 12, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(a->origin.data(), req.origin.p, n * 8, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(a->rowpos.data(), req.rowpos.p, n * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return cleanup(e);
+    }
+    if (req.jacobian) {
+        // keep the patches and origins on the device; row range of every frame
+        std::swap(a->d_jac.p, req.jac.p);
+        std::swap(a->d_jac.bytes, req.jac.bytes);
+        std::swap(a->d_origin.p, req.origin.p);
+        std::swap(a->d_origin.bytes, req.origin.bytes);
+        std::vector<uint64_t> fr(n_frames + 1, 0);
+        for (uint64_t r = 0; r < n; ++r) fr[(size_t)a->rowpos[r] + 1] += 1;
+        for (uint64_t f = 0; f < n_frames; ++f) fr[f + 1] += fr[f];
+        e = a->d_frame_rows.ensure(fr.size() * 8);
+        if (e == hipSuccess) e = hipMemcpy(a->d_frame_rows.p, fr.data(), fr.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(e);
+    }
+    *out = a;
+    return GT_OK;
+}
+
+uint64_t gt_adjoint_rows(const gt_adjoint* a) { return a ? a->fwd->rows : 0; }
+uint32_t gt_adjoint_ncols(const gt_adjoint* a) { return a ? a->fwd->ncols : 0; }
+uint32_t gt_adjoint_patch(const gt_adjoint* a) { return a ? a->S : 0; }
+const float* gt_adjoint_data(const gt_adjoint* a) {
+    return a && !a->fwd->data.empty() ? a->fwd->data.data() : nullptr;
+}
+const int64_t* gt_adjoint_candidates(const gt_adjoint* a) {
+    return a && !a->fwd->cand.empty() ? a->fwd->cand.data() : nullptr;
+}
+const float* gt_adjoint_sigma(const gt_adjoint* a) {
+    return a && !a->sigma.empty() ? a->sigma.data() : nullptr;
+}
+const int32_t* gt_adjoint_origin(const gt_adjoint* a) {
+    return a && !a->origin.empty() ? a->origin.data() : nullptr;
+}
+
+gt_status gt_adjoint_jacobian(const gt_adjoint* a, float* out, uint64_t capacity_floats) {
+    if (!a || !out) return fail(GT_E_INVALID, "NULL argument");
+    if (!a->jacobian) return fail(GT_E_INVALID, "the adjoint was computed without GT_ADJ_JACOBIAN");
+    const uint64_t n = a->fwd->rows * 3ull * a->S * a->S;
+    if (capacity_floats < n) return fail(GT_E_INVALID, "capacity too small");
+    if (n) HIP_TRY(hipMemcpy(out, a->d_jac.p, n * 4, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad_rows,
+                         float* grad_frames, uint32_t flags) {
+    if (!ctx || !a) return fail(GT_E_INVALID, "NULL argument");
+    if (!a->jacobian) return fail(GT_E_INVALID, "the adjoint was computed without GT_ADJ_JACOBIAN");
+    if (!grad_frames && a->nframes) return fail(GT_E_INVALID, "grad_frames is NULL");
+    const uint64_t n = a->fwd->rows;
+    if (!grad_rows && n) return fail(GT_E_INVALID, "grad_rows is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const bool on_device = flags & GT_FRAMES_ON_DEVICE;
+    const uint64_t gbytes = a->nframes * (uint64_t)a->h * a->w * 4;
+    struct ScopedBuf : DevBuf {
+        ~ScopedBuf() { release(); }
+    } dg, dgrad;
+    const float* grows = grad_rows;
+    float* gout = grad_frames;
+    if (!on_device) {
+        HIP_TRY(dg.ensure(n * 12));
+        HIP_TRY(dgrad.ensure(gbytes));
+        if (n) HIP_TRY(hipMemcpy(dg.p, grad_rows, n * 12, hipMemcpyHostToDevice));
+        grows = dg.as<float>();
+        gout = dgrad.as<float>();
+    }
+    hipStream_t s = ctx->streams[0];
+    hipError_t e = launch_adjoint_vjp(a->d_jac.as<float>(), a->d_origin.as<int32_t>(),
+                                      a->d_frame_rows.as<uint64_t>(), (uint32_t)a->nframes, grows,
+                                      gout, (int)a->S, (int)a->h, (int)a->w, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && !on_device && gbytes)
+        e = hipMemcpy(grad_frames, gout, gbytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(GT_E_HIP, "gt_adjoint_vjp: %s", hipGetErrorString(e));
+    return GT_OK;
+}
+
+void gt_adjoint_free(gt_adjoint* a) { delete a; }
 
 uint64_t gt_result_rows(const gt_result* r) { return r ? r->rows : 0; }
 uint32_t gt_result_ncols(const gt_result* r) { return r ? r->ncols : 0; }

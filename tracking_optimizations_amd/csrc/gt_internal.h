@@ -114,6 +114,14 @@ struct Batch {
     uint64_t rows_cap;
 };
 
+// Adjoint outputs, indexed by output row (gt_adjoint.hip).
+struct AdjOut {
+    float* sigma;          // [rows_cap * 3] sigma_y, sigma_x, sigma_mass
+    int32_t* origin;       // [rows_cap * 2] patch origin (row, column)
+    int32_t* rowpos;       // [rows_cap] position of the row's frame in the call
+    float* jac;            // [rows_cap * 3 * S * S] Jacobian patches, or null
+};
+
 // Launchers (gt_kernels.hip).  All launch on `s`; return hipSuccess or the
 // first launch error.
 hipError_t launch_zero_batch(const Batch& b, hipStream_t s);
@@ -139,6 +147,15 @@ hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
                        hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+
+// Adjoint (gt_adjoint.hip).
+int adjoint_patch(const Geom& g);
+bool adjoint_supported(const Geom& g);
+hipError_t launch_adjoint(const Batch& b, const Geom* d_geom, const AdjOut& ao,
+                          uint32_t frame_pos0, hipStream_t s);
+hipError_t launch_adjoint_vjp(const float* jac, const int32_t* origin, const uint64_t* frame_rows,
+                              uint32_t nframes, const float* grad_rows, float* grad, int S, int H,
+                              int W, hipStream_t s);
 
 // Synthetic frames (gt_synth.hip).
 hipError_t launch_synth(uint16_t* frames, uint64_t nframes, uint32_t h,
