@@ -75,6 +75,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
                     help="locate without the percentile test (the reference's default)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner (rows per bandpass item, frames "
+                         "per launch)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
@@ -138,6 +141,13 @@ def main() -> int:
         log(f"copy calibration: k_copy read {nb} bytes and wrote {nb} bytes")
         dst.free()
 
+    tuning = None
+    if not args.no_autotune:
+        tuning = ctx.autotune(params, H, W, n_frames=min(nf, 64))
+        log(f"autotune: {tuning['seg_rows']} rows per bandpass item, "
+            f"{tuning['frames_per_launch'] or 'automatic'} frames per launch, "
+            f"{1000 * tuning['device_ms_per_frame']:.1f} us/frame on the tuning sample")
+
     for i in range(args.warmup):
         t = run(batched_flags)
         log(f"warmup {i}: device {t['device_ms']:.1f} ms, {t['candidates']} candidates, "
@@ -170,6 +180,17 @@ def main() -> int:
     sync()
     fps_noev = nf * args.steps / (time.perf_counter() - t1)
     log(f"batched without stage events: {fps_noev:.0f} frames/s")
+    # the same steps replaying the captured launch sequence
+    ctx.set_graphs(True)
+    run(batched_flags)  # capture
+    sync()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        run(batched_flags)
+    sync()
+    fps_graph = nf * args.steps / (time.perf_counter() - t1)
+    ctx.set_graphs(False)
+    log(f"batched, graph replay: {fps_graph:.0f} frames/s")
     for k, (ms, n) in stages.items():
         if n:
             log(f"  stage {k:9s}: {ms:8.3f} ms/step over {n} launches ({ms / n:.4f} ms/launch)")
@@ -246,6 +267,8 @@ def main() -> int:
                      "device_ms_per_step": dev_ms,
                      "stage_ms": {k: v[0] for k, v in stages.items() if v[1]}},
         "frames_per_s_without_stage_events": fps_noev,
+        "frames_per_s_graph_replay": fps_graph,
+        "tuning": tuning,
         "plain": plain,
         "batched_over_plain": (fps / plain["frames_per_s"]) if plain else None,
         "cpu_baseline": cpu,

@@ -145,6 +145,24 @@ gt_status gt_context_destroy(gt_context* ctx);
 gt_status gt_context_timing(const gt_context* ctx, gt_timing* out);
 /* Frames per launch of the batched path (0 = automatic). */
 gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch);
+/* 1: the batched path's launch sequence is captured as a HIP graph and
+ * replayed while calls repeat with the same inputs, parameters and buffers
+ * (a device staging buffer refilled between calls, a benchmark loop).  The
+ * graph holds no memset node.  0 (default): launches are issued directly. */
+gt_status gt_context_set_graphs(gt_context* ctx, int enable);
+
+/* Autotuner: times the batched path on n_frames synthetic uint16 frames of
+ * h x w for every launch shape (rows per bandpass work item 64/128/256,
+ * frames per launch automatic/16/32) and keeps the fastest in the context
+ * for calls whose frames are all h x w with the same filter geometry. */
+typedef struct gt_tuning {
+    uint32_t seg_rows;            /* rows per bandpass work item */
+    uint32_t frames_per_launch;   /* 0 = as many as the workspace holds */
+    double device_ms_per_frame;   /* of the chosen shape */
+} gt_tuning;
+gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, uint32_t w,
+                              uint32_t n_frames, gt_tuning* out);
+
 /* 1 (default): where the geometry allows it, the centroid walk runs inside
  * the detect kernel on the tile it has staged; 0: a separate refine kernel
  * walks every candidate.  Results are identical either way. */

@@ -180,3 +180,31 @@ def test_walk_in_detect_equals_refine_kernel(frames512, case):
     a = locate(frames512, p, ctx=fused)
     b = locate(frames512, p, ctx=separate)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_graph_replay_matches_direct_launches(frames512):
+    """Replays of the captured launch sequence, with direct launches and
+    synchronisations in between, give the same rows every time."""
+    p = make_params(**CASES["c2"])
+    movie = np.ascontiguousarray(np.concatenate([frames512] * 3))
+    ref = locate(movie, p, ctx=_lib.Context(0))
+    gctx = _lib.Context(0)
+    gctx.set_batch(2)           # several batches on both streams
+    gctx.set_graphs(True)
+    other = _lib.Context(0)
+    for rep in range(4):
+        print(f"[parity] graph replay {rep}", flush=True)
+        got = locate(movie, p, ctx=gctx)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), rep
+        locate(frames512[:1], make_params(**CASES["c1"]), ctx=other)  # eager work between
+        other.synchronize()
+
+
+def test_autotune_keeps_results(frames512):
+    p = make_params(**CASES["c2"])
+    ref = locate(frames512, p, ctx=_lib.Context(0))
+    ctx = _lib.Context(0)
+    t = ctx.autotune(p, 512, 512, n_frames=8)
+    assert t["seg_rows"] in (64, 128, 256) and t["frames_per_launch"] in (0, 16, 32)
+    got = locate(frames512, p, ctx=ctx)
+    assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])

@@ -34,7 +34,7 @@ EXPORTS = [
     "gt_params_default", "gt_params_rust_default", "gt_params_validate", "gt_column_names",
     "gt_last_error", "gt_version", "gt_device_count", "gt_context_create",
     "gt_context_destroy", "gt_context_timing", "gt_context_set_batch",
-    "gt_context_set_fused_walk", "gt_locate_u16",
+    "gt_context_set_fused_walk", "gt_context_set_graphs", "gt_context_autotune", "gt_locate_u16",
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
@@ -112,6 +112,11 @@ class GtTiming(ctypes.Structure):
     ]
 
 
+class GtTuning(ctypes.Structure):
+    _fields_ = [("seg_rows", ctypes.c_uint32), ("frames_per_launch", ctypes.c_uint32),
+                ("device_ms_per_frame", ctypes.c_double)]
+
+
 class GtFrameDesc(ctypes.Structure):
     _fields_ = [
         ("data", ctypes.c_void_p),
@@ -152,6 +157,9 @@ def load() -> ctypes.CDLL:
         L.gt_context_timing.argtypes = [vp, P(GtTiming)]
         L.gt_context_set_batch.argtypes = [vp, ctypes.c_uint32]
         L.gt_context_set_fused_walk.argtypes = [vp, ctypes.c_int]
+        L.gt_context_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.gt_context_autotune.argtypes = [vp, P(GtParams), ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint32, P(GtTuning)]
         for name in ("gt_locate_u16", "gt_locate_u8", "gt_locate_f32"):
             f = getattr(L, name)
             f.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(GtParams),
@@ -259,6 +267,16 @@ class Context:
 
     def set_fused_walk(self, enable: bool) -> None:
         check(load().gt_context_set_fused_walk(self.handle, 1 if enable else 0))
+
+    def set_graphs(self, enable: bool) -> None:
+        check(load().gt_context_set_graphs(self.handle, 1 if enable else 0))
+
+    def autotune(self, params, h: int, w: int, n_frames: int = 32) -> dict:
+        t = GtTuning()
+        check(load().gt_context_autotune(self.handle, ctypes.byref(params), h, w, n_frames,
+                                         ctypes.byref(t)))
+        return {"seg_rows": t.seg_rows, "frames_per_launch": t.frames_per_launch,
+                "device_ms_per_frame": t.device_ms_per_frame}
 
     def synchronize(self) -> None:
         check(load().gt_synchronize(self.handle))

@@ -2,6 +2,7 @@
 // persistent device context, batch planning and the C ABI of
 // include/gpu_tracking.h.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -273,6 +274,15 @@ struct gt_context {
     size_t workspace_limit = (size_t)6 << 30;   // per workspace
     int seg_rows = 128;
     int fuse_walk = 1;     // run the centroid walk inside detect where supported
+    // batched launch sequence captured as a graph and replayed while the
+    // call's inputs, parameters and buffers are unchanged
+    int graphs = 0;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> gkey;
+    uint64_t glaunches = 0;
+    // autotuned (rows per bandpass work item, frames per launch) per frame
+    // shape and filter geometry
+    std::map<std::array<int64_t, 6>, std::pair<int, uint32_t>> tuned;
     // per-stage event timing (forces one stream so that kernels do not overlap)
     int profiling = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -334,6 +344,10 @@ struct FrameSizes {
     uint32_t k1_tx, k1_ty, k1_items;
     uint32_t tx, ty, tiles;
 };
+
+std::array<int64_t, 6> tune_key(uint32_t h, uint32_t w, const Geom& g) {
+    return {(int64_t)h, (int64_t)w, g.R, g.kd, g.rint, g.has_pct};
+}
 
 FrameSizes sizes_of(uint32_t h, uint32_t w, int seg) {
     FrameSizes s;
@@ -445,9 +459,22 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     Geom g;
     GT_TRY(make_geom(p, integer_input, g));
     g.walk_in_detect = ctx->fuse_walk && walk_in_detect_supported(g) ? 1 : 0;
+    // autotuned launch shape for calls whose frames all share one shape
+    int seg = ctx->seg_rows;
+    uint32_t batch_frames = ctx->batch_frames;
+    if (!frames.empty() && ctx->batch_frames == 0) {
+        bool uniform = true;
+        for (const auto& f : frames) uniform = uniform && f.h == frames[0].h && f.w == frames[0].w;
+        if (uniform) {
+            auto it = ctx->tuned.find(tune_key(frames[0].h, frames[0].w, g));
+            if (it != ctx->tuned.end()) {
+                seg = it->second.first;
+                batch_frames = it->second.second;
+            }
+        }
+    }
     if (adj && !adjoint_supported(g))
         return fail(GT_E_NOT_IMPLEMENTED, "adjoint needs (2*rint+1)^2 <= 1024 and smoothing_size/2 <= 16");
-    const int seg = ctx->seg_rows;
     const size_t es = elem_size(in_type);
     // vector loads need W % 4 == 0 and frame bases aligned to 4 pixels
     bool vec = true;
@@ -484,7 +511,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     }
 
     // ---- batch plan -------------------------------------------------------
-    const size_t max_batch = plain ? 1 : (ctx->batch_frames ? ctx->batch_frames : 4096);
+    const size_t max_batch = plain ? 1 : (batch_frames ? batch_frames : 4096);
     std::vector<BatchPlan> plan;
     Need maxneed;
     {
@@ -629,14 +656,14 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         launches = 0;
 
         hipStream_t s0 = ctx->streams[0];
-        HIP_TRY(hipEventRecord(ctx->ev0, s0));
+        uint64_t host_rows_done = 0;
+        auto enqueue = [&]() -> gt_status {
         HIP_TRY(launch_fill_u64(counters, 2, 0, s0));
         ++launches;
         if (nstreams > 1) {
             HIP_TRY(hipEventRecord(ctx->ev_join, s0));
             HIP_TRY(hipStreamWaitEvent(ctx->streams[1], ctx->ev_join, 0));
         }
-        uint64_t host_rows_done = 0;
         for (size_t bi = 0; bi < plan.size(); ++bi) {
             const auto& bp = plan[bi];
             const int si = (int)(bi % nstreams);
@@ -757,6 +784,70 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         if (nstreams > 1) {
             HIP_TRY(hipEventRecord(ctx->ev_join, ctx->streams[1]));
             HIP_TRY(hipStreamWaitEvent(s0, ctx->ev_join, 0));
+        }
+        return GT_OK;
+        };  // enqueue
+
+        // Replay the captured launch sequence when this call repeats the
+        // previous one (same inputs, parameters, launch shape and buffers);
+        // capture it otherwise.  Any capture failure falls back to launching
+        // directly and turns graphs off for the context.
+        bool launched = false;
+        if (ctx->graphs && !plain && !ctx->profiling && !adj) {
+            std::vector<uint64_t> key;
+            for (const auto& f : frames) {
+                key.push_back((uint64_t)(uintptr_t)f.ptr);
+                key.push_back(((uint64_t)f.h << 32) | f.w);
+                key.push_back((uint64_t)f.fi);
+            }
+            const uint64_t* pw = (const uint64_t*)p;
+            for (size_t i = 0; i < sizeof(gt_params) / 8; ++i) key.push_back(pw[i]);
+            key.push_back(((uint64_t)flags << 32) | (uint32_t)in_type);
+            key.push_back(rows_cap);
+            key.push_back(((uint64_t)seg << 32) | (uint32_t)nstreams);
+            key.push_back(((uint64_t)batch_frames << 32) | (uint32_t)g.walk_in_detect);
+            const DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->rows,
+                                    &ctx->rows_cand, &ctx->counters};
+            for (const DevBuf* d : bufs) key.push_back((uint64_t)(uintptr_t)d->p);
+            for (int si = 0; si < nstreams; ++si) {
+                const Workspace& w = ctx->ws[si];
+                const DevBuf* wb[] = {&w.in_stage, &w.P, &w.tmp, &w.list, &w.slots,
+                                      &w.rec, &w.keep, &w.tile3, &w.perframe};
+                for (const DevBuf* d : wb) key.push_back((uint64_t)(uintptr_t)d->p);
+            }
+            if (!(ctx->gexec && ctx->gkey == key)) {
+                if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+                ctx->gexec = nullptr;
+                ctx->gkey.clear();
+                if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) == hipSuccess) {
+                    const gt_status st = enqueue();
+                    hipGraph_t gr = nullptr;
+                    const hipError_t e = hipStreamEndCapture(s0, &gr);
+                    hipGraphExec_t ex = nullptr;
+                    if (st == GT_OK && e == hipSuccess && gr &&
+                        hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess) {
+                        ctx->gexec = ex;
+                        ctx->gkey = key;
+                        ctx->glaunches = launches;
+                    }
+                    if (gr) (void)hipGraphDestroy(gr);
+                }
+                if (!ctx->gexec) {
+                    ctx->graphs = 0;
+                    (void)hipGetLastError();
+                }
+            }
+            if (ctx->gexec) {
+                HIP_TRY(hipEventRecord(ctx->ev0, s0));
+                HIP_TRY(hipGraphLaunch(ctx->gexec, s0));
+                launches = ctx->glaunches;
+                launched = true;
+            }
+        }
+        if (!launched) {
+            launches = 0;
+            HIP_TRY(hipEventRecord(ctx->ev0, s0));
+            GT_TRY(enqueue());
         }
         HIP_TRY(hipEventRecord(ctx->ev1, s0));
         uint64_t cnt[2] = {0, 0};
@@ -927,6 +1018,7 @@ gt_status gt_context_destroy(gt_context* ctx) {
     if (!ctx) return GT_OK;
     (void)hipSetDevice(ctx->device);
     for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(ctx->streams[i]);
+    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->rows, &ctx->rows_cand,
                       &ctx->counters};
     for (DevBuf* b : bufs) b->release();
@@ -965,6 +1057,103 @@ gt_status gt_context_stage_times(const gt_context* ctx, double* ms, uint64_t* la
     for (int i = 0; i < GT_NSTAGES; ++i) {
         if (ms) ms[i] = ctx->stage_ms[i];
         if (launches) launches[i] = ctx->stage_launches[i];
+    }
+    return GT_OK;
+}
+
+gt_status gt_context_set_graphs(gt_context* ctx, int enable) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    ctx->graphs = enable ? 1 : 0;
+    if (!enable && ctx->gexec) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipGraphExecDestroy(ctx->gexec);
+        ctx->gexec = nullptr;
+        ctx->gkey.clear();
+    }
+    return GT_OK;
+}
+
+gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, uint32_t w,
+                              uint32_t n_frames, gt_tuning* out) {
+    if (!ctx || !p) return fail(GT_E_INVALID, "NULL argument");
+    GT_TRY(check_params(p, h, w));
+    if (n_frames == 0) n_frames = 1;
+    HIP_TRY(hipSetDevice(ctx->device));
+    Geom g;
+    GT_TRY(make_geom(p, true, g));
+    // synthetic uint16 frames: background 100 + Poisson, one spot per 800 px^2
+    const uint64_t npx = (uint64_t)h * w;
+    struct ScopedBuf : DevBuf {
+        ~ScopedBuf() { release(); }
+    } movie;
+    HIP_TRY(movie.ensure(npx * n_frames * 2));
+    {
+        uint64_t st = 0x9e3779b97f4a7c15ull ^ ((uint64_t)h << 20) ^ w;
+        auto rnd = [&]() {  // splitmix64 -> [0, 1)
+            uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+            return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+        };
+        const uint64_t per = std::max<uint64_t>(1, npx / 800);
+        std::vector<float> spots;
+        std::vector<uint64_t> off(n_frames + 1, 0);
+        std::vector<float> bg(n_frames, 100.f);
+        for (uint32_t f = 0; f < n_frames; ++f) {
+            for (uint64_t i = 0; i < per; ++i) {
+                spots.push_back((float)(rnd() * (h - 1)));
+                spots.push_back((float)(rnd() * (w - 1)));
+                spots.push_back((float)(500.0 + 1500.0 * rnd()));
+                spots.push_back((float)(1.5 + rnd()));
+            }
+            off[f + 1] = off[f] + per;
+        }
+        GT_TRY(gt_synth_u16(ctx, movie.as<uint16_t>(), n_frames, h, w, spots.data(), off.data(),
+                            bg.data(), 7));
+    }
+    const int saved_seg = ctx->seg_rows;
+    const uint32_t saved_batch = ctx->batch_frames;
+    const int saved_graphs = ctx->graphs;
+    ctx->graphs = 0;
+    ctx->tuned.erase(tune_key(h, w, g));
+    const int segs[] = {64, 128, 256};
+    const uint32_t batches[] = {0, 16, 32};
+    double best = 1e30;
+    int best_seg = saved_seg;
+    uint32_t best_batch = 0;
+    gt_status st = GT_OK;
+    for (int sg : segs) {
+        if (g.has_pct && (int64_t)K1_STRIP * sg >= 65536) continue;
+        for (uint32_t bf : batches) {
+            if (bf && bf >= n_frames) continue;
+            ctx->seg_rows = sg;
+            ctx->batch_frames = bf;
+            double ms = 1e30;
+            for (int rep = 0; rep < 3 && st == GT_OK; ++rep) {
+                gt_result* r = nullptr;
+                st = gt_locate_u16(ctx, movie.as<uint16_t>(), n_frames, h, w, p,
+                                   GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE, &r);
+                gt_result_free(r);
+                if (rep > 0) ms = std::min(ms, (double)ctx->last.device_ms);
+            }
+            if (st != GT_OK) break;
+            if (ms < best) {
+                best = ms;
+                best_seg = sg;
+                best_batch = bf;
+            }
+        }
+        if (st != GT_OK) break;
+    }
+    ctx->seg_rows = saved_seg;
+    ctx->batch_frames = saved_batch;
+    ctx->graphs = saved_graphs;
+    if (st != GT_OK) return st;
+    ctx->tuned[tune_key(h, w, g)] = {best_seg, best_batch};
+    if (out) {
+        out->seg_rows = (uint32_t)best_seg;
+        out->frames_per_launch = best_batch;
+        out->device_ms_per_frame = best / n_frames;
     }
     return GT_OK;
 }
