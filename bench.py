@@ -75,9 +75,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
                     help="locate without the percentile test (the reference's default)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the launch-shape autotuner (rows per bandpass item, frames "
-                         "per launch)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-shape autotuner (rows per bandpass item, frames per "
+                         "launch) before timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
@@ -142,7 +142,7 @@ def main() -> int:
         dst.free()
 
     tuning = None
-    if not args.no_autotune:
+    if args.autotune:
         tuning = ctx.autotune(params, H, W, n_frames=min(nf, 64))
         log(f"autotune: {tuning['seg_rows']} rows per bandpass item, "
             f"{tuning['frames_per_launch'] or 'automatic'} frames per launch, "
@@ -201,7 +201,9 @@ def main() -> int:
     # algorithmic bytes of each kernel over the whole step
     kernel_bytes = {
         "bandpass": px * (2 + 4),                       # u16 in, float32 out
-        "detect": px * 4 + n_cand * 2,                  # float32 in, candidate slots out
+        # float32 in, candidate slots out, plus (centroid walk fused in
+        # detect) one disk box read and one record written per candidate
+        "detect": px * 4 + n_cand * (2 + 4 * (2 * rint + 1) ** 2 + 16),
         "refine": n_cand * (2 + 4 * (2 * rint + 1) ** 2 + 12),
         "dedup": n_cand * 16.0,
         "emit": n_rows * 4.0 * 4 + n_rows * 8.0,
@@ -209,6 +211,16 @@ def main() -> int:
     dom_bytes = kernel_bytes.get(dominant, 0.0)
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     per_launch_bytes = dom_bytes / max(dom_n, 1)
+    # measured HBM bytes of the dominant kernel (rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE per dispatch, scaled by a copy kernel of known size; see
+    # profiles/traffic_movie_2048.json) for this launch's frame count
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_movie_2048.json")
+    if H == 2048 and W == 2048 and os.path.exists(tpath):
+        tk = json.load(open(tpath)).get("kernels", {})
+        for name, rec in tk.items():
+            if name.startswith(f"gt::k_{dominant}") and rec.get("read_bytes_per_frame") is not None:
+                traffic = (rec["read_bytes_per_frame"] + rec["write_bytes_per_frame"]) * nf / max(dom_n, 1)
     pipe_bytes = algorithmic_bytes(H, W, n_cand / nf, n_rows / nf, 4, rint) * nf
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
     log(f"dominant kernel {dominant}: {achieved:.0f} GB/s over {per_launch_bytes / 1e6:.1f} MB "
@@ -260,7 +272,7 @@ def main() -> int:
                    "params": kw, "path": "batched", "frames": nf},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": per_launch_bytes,
+                     "traffic": traffic, "bytes_per_launch": per_launch_bytes,
                      "ms_per_launch": dom_ms / max(dom_n, 1)},
         "pipeline": {"algorithmic_GBps": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS,
                      "candidates_per_frame": n_cand / nf, "rows_per_frame": n_rows / nf,

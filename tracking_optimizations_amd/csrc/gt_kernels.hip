@@ -1133,6 +1133,7 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 __global__ __launch_bounds__(1024) void k_select2(Batch b, float percentile) {
     __shared__ uint32_t s_ha[2][2048];
     __shared__ uint32_t s_res[2][3];
+    __shared__ uint32_t s_wsum[2][16];
     const uint32_t f = blockIdx.x;
     const Frame fr = b.frames[f];
     const uint64_t n = (uint64_t)fr.H * fr.W;
@@ -1140,6 +1141,7 @@ __global__ __launch_bounds__(1024) void k_select2(Batch b, float percentile) {
     double frac;
     pct_ranks(n, percentile, lo, hi, frac);
     const int t = threadIdx.x;
+    if (t < 6) s_res[t / 3][t % 3] = 0;
     const int bin[2] = {b.selstate[f * 4 + 0], b.selstate[f * 4 + 2]};
     const int rank[2] = {b.selstate[f * 4 + 1], b.selstate[f * 4 + 3]};
     const bool need[2] = {bin[0] != 0, hi != lo && bin[1] != 0};
@@ -1172,21 +1174,41 @@ __global__ __launch_bounds__(1024) void k_select2(Batch b, float percentile) {
             }
         }
         __syncthreads();
-        const bool need_t = t == 0 ? need[0] : need[1];
-        if (t < 2 && need_t) {
-            const int bin_t = t == 0 ? bin[0] : bin[1];
-            const int rank_t = t == 0 ? rank[0] : rank[1];
-            const uint32_t want = pass == 0 ? (uint32_t)rank_t : s_res[t][1];
-            uint32_t cum = 0, sel = 0;
-            for (int i = 0; i < nb; ++i) {
-                if (want < cum + s_ha[t][i]) { sel = (uint32_t)i; break; }
-                cum += s_ha[t][i];
-            }
-            if (pass == 0) {
-                s_res[t][0] = sel;
-                s_res[t][1] = want - cum;
-            } else {
-                s_res[t][2] = ((uint32_t)(bin_t - 1) << 19) | (s_res[t][0] << 8) | sel;
+        // Selection: a workgroup-wide exclusive prefix sum over the bins
+        // (thread t holds bins 2t, 2t+1 of 2048, or bin t of 256); the thread
+        // whose range holds the wanted rank picks the bin.
+        const uint32_t want0 = pass == 0 ? (uint32_t)rank[0] : s_res[0][1];
+        const uint32_t want1 = pass == 0 ? (uint32_t)rank[1] : s_res[1][1];
+        uint32_t c0a, c0b, c1a, c1b;
+        if (nb == 2048) {
+            c0a = s_ha[0][2 * t]; c0b = s_ha[0][2 * t + 1];
+            c1a = s_ha[1][2 * t]; c1b = s_ha[1][2 * t + 1];
+        } else {
+            c0a = t < 256 ? s_ha[0][t] : 0u; c0b = 0u;
+            c1a = t < 256 ? s_ha[1][t] : 0u; c1b = 0u;
+        }
+        const int lane = t & 63, wv = t >> 6;
+        const uint32_t l0 = c0a + c0b, l1 = c1a + c1b;
+        const uint32_t i0 = wave_incl_scan(l0, lane), i1 = wave_incl_scan(l1, lane);
+        if (lane == 63) { s_wsum[0][wv] = i0; s_wsum[1][wv] = i1; }
+        __syncthreads();
+        uint32_t e0 = i0 - l0, e1 = i1 - l1;
+        for (int q = 0; q < wv; ++q) { e0 += s_wsum[0][q]; e1 += s_wsum[1][q]; }
+        const int per = nb == 2048 ? 2 : 1;
+#pragma unroll
+        for (int sidx = 0; sidx < 2; ++sidx) {
+            const uint32_t want = sidx ? want1 : want0;
+            const uint32_t ex = sidx ? e1 : e0, ca = sidx ? c1a : c0a, cb = sidx ? c1b : c0b;
+            if (need[sidx] && want >= ex && want < ex + ca + cb) {
+                const bool first = want < ex + ca;
+                const uint32_t sel = (uint32_t)(per * t) + (first ? 0u : 1u);
+                const uint32_t below = first ? ex : ex + ca;
+                if (pass == 0) {
+                    s_res[sidx][0] = sel;
+                    s_res[sidx][1] = want - below;
+                } else {
+                    s_res[sidx][2] = ((uint32_t)(bin[sidx] - 1) << 19) | (s_res[sidx][0] << 8) | sel;
+                }
             }
         }
         __syncthreads();
@@ -1722,39 +1744,51 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     const float margin = rad + 1.0f;
     bymin -= margin; bymax += margin; bxmin -= margin; bxmax += margin;
 
-    // stage the neighbourhood (wave-uniform loops; lanes take records)
+    // stage the neighbourhood: the counts of all neighbour tiles are read in
+    // one round (lane q holds tile q), then the records of all tiles are
+    // enumerated 64 at a time, so the loads of a round are independent
     uint32_t S = 0;
     bool staged = ntl <= 64;
     if (staged) {
-        for (int q = 0; q < ntl && staged; ++q) {
-            const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
-            const uint32_t it2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-            const uint32_t c2 = b.tile_count[it2];
-            const size_t s2 = (size_t)it2 * TILE_PX;
-            for (uint32_t jb = 0; jb < c2; jb += 64) {
-                const uint32_t j = jb + lane;
-                bool take = false;
-                float m = 0.f, y = 0.f, x = 0.f;
-                if (j < c2) {
-                    m = rmass[s2 + j];
-                    y = ry[s2 + j];
-                    x = rx[s2 + j];
-                    take = ok(m, s2 + j) && y >= bymin && y <= bymax && x >= bxmin &&
-                           x <= bxmax;
-                }
-                const uint64_t tm = __ballot(take);
-                const uint32_t nt = (uint32_t)__popcll(tm);
-                if (S + nt > (uint32_t)DD_CAP) { staged = false; break; }
-                if (take) {
-                    const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
-                    const uint32_t l2 = slots[s2 + j];
-                    s_m[wave][r] = m;
-                    s_y[wave][r] = y;
-                    s_x[wave][r] = x;
-                    s_i[wave][r] = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
-                }
-                S += nt;
+        uint32_t myc = 0, myit = 0;
+        if (lane < ntl) {
+            const int ny = y_lo + lane / nxr, nx = x_lo + lane % nxr;
+            myit = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+            myc = b.tile_count[myit];
+        }
+        const uint32_t inc = wave_incl_scan(myc, lane);
+        const uint32_t exq = inc - myc;
+        const uint32_t T = __shfl(inc, 63, 64);
+        for (uint32_t jb = 0; jb < T && staged; jb += 64) {
+            const uint32_t j = jb + lane;
+            // tile of record j: the last q whose first record is <= j
+            int q = 0;
+            for (int qq = 1; qq < ntl; ++qq)
+                if (__shfl(exq, qq, 64) <= j) q = qq;
+            const uint32_t it2 = __shfl(myit, q, 64);
+            const uint32_t i2 = j - __shfl(exq, q, 64);
+            const size_t s2 = (size_t)it2 * TILE_PX + i2;
+            bool take = false;
+            float m = 0.f, y = 0.f, x = 0.f;
+            if (j < T) {
+                m = rmass[s2];
+                y = ry[s2];
+                x = rx[s2];
+                take = ok(m, s2) && y >= bymin && y <= bymax && x >= bxmin && x <= bxmax;
             }
+            const uint64_t tm = __ballot(take);
+            const uint32_t nt = (uint32_t)__popcll(tm);
+            if (S + nt > (uint32_t)DD_CAP) { staged = false; break; }
+            if (take) {
+                const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
+                const uint32_t l2 = slots[s2];
+                const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
+                s_m[wave][r] = m;
+                s_y[wave][r] = y;
+                s_x[wave][r] = x;
+                s_i[wave][r] = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
+            }
+            S += nt;
         }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
