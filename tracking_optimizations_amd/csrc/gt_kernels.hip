@@ -648,8 +648,6 @@ __device__ __forceinline__ void sliding_max(const float (&x)[N + K - 1], float (
     for (int i = 0; i < N; ++i) y[i] = fmaxf(S[i], Pm[i + K - 1]);
 }
 
-constexpr int DET_CAND_CAP = 1024;   // candidate pixels kept in LDS per tile
-
 // Centroid walk of the tile's candidates (centers.wgsl:23-109 and the walk
 // of walk_cols.wgsl, as in k_refine): candidate k of the tile (raster
 // order) walks in lane k % 256 of the workgroup, so a typical tile (a few
@@ -658,7 +656,6 @@ constexpr int DET_CAND_CAP = 1024;   // candidate pixels kept in LDS per tile
 template <int RI, int LR, int LC, int LS>
 __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                                           const float* __restrict__ L,
-                                          const uint16_t* __restrict__ s_cand,
                                           const GLOBAL uint16_t* __restrict__ slots,
                                           uint32_t total, uint32_t item,
                                           const GLOBAL float* __restrict__ P, int H, int W,
@@ -670,7 +667,7 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
     const uint64_t cap = dp.slot_cap;
     float* rec = b.rec;
     for (uint32_t k = (uint32_t)tid; k < total; k += 256) {
-        const uint32_t l = k < (uint32_t)DET_CAND_CAP ? s_cand[k] : slots[k];
+        const uint32_t l = slots[k];  // written by this workgroup before the barrier
         const int lr0 = (int)(l / TILE_W), lc0 = (int)(l % TILE_W);
         int u = y0 + lr0, v = x0 + lc0;
         float m = 0.f, c0 = 0.f, c1 = 0.f;
@@ -759,10 +756,9 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int NRUN = (LR * 4 + 255) / 256;
     static_assert(LC - 64 <= 64, "halo wider than one extra column pass");
     __shared__ __attribute__((aligned(16))) float L[LR * LS];
+    // candidates per tile row, then (in place) the row's first slot
     __shared__ uint32_t s_rowcnt[TILE_H];
-    __shared__ uint32_t s_off[TILE_H];
     __shared__ uint32_t s_lcnt, s_total;
-    __shared__ uint16_t s_cand[RI > 0 ? DET_CAND_CAP : 1];
 
     const int tid = threadIdx.x;
     const int a = dp.a;
@@ -929,7 +925,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     if (tid < 64) {
         const uint32_t cnt = s_rowcnt[tid];
         const uint32_t inc = wave_incl_scan(cnt, tid);
-        s_off[tid] = inc - cnt;
+        s_rowcnt[tid] = inc - cnt;
         if (tid == 63) {
             b.tile_count[item] = inc;
             s_total = inc;
@@ -943,12 +939,9 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         const int r = kw * N + m;
         if ((bal[m] >> c) & 1ull) {
             const uint32_t rank =
-                s_off[r] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[m] >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bal[m], 0u));
+                s_rowcnt[r] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[m] >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal[m], 0u));
             slots[rank] = (uint16_t)(r * TILE_W + c);
-            if constexpr (RI > 0) {
-                if (rank < (uint32_t)DET_CAND_CAP) s_cand[rank] = (uint16_t)(r * TILE_W + c);
-            }
         }
     }
 
@@ -958,7 +951,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         __syncthreads();         // column pass done with L; slots visible
         stage_tile(false);       // columns 0..63 again hold P
         __syncthreads();
-        walk_tile<RI, LR, LC, LS>(b, dp, L, s_cand, slots, total, item, P, H, W, x0, y0, a, tid);
+        walk_tile<RI, LR, LC, LS>(b, dp, L, slots, total, item, P, H, W, x0, y0, a, tid);
     }
 }
 
