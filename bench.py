@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
                     help="locate without the percentile test (the reference's default)")
+    ap.add_argument("--separate-refine", action="store_true",
+                    help="walk in a separate refine kernel instead of inside detect")
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-shape autotuner (rows per bandpass item, frames per "
                          "launch) before timing")
@@ -106,6 +108,8 @@ def main() -> int:
     ctx = _lib.Context(0)
     if args.batch:
         ctx.set_batch(args.batch)
+    if args.separate_refine:
+        ctx.set_fused_walk(False)
     L = _lib.load()
     import ctypes
 

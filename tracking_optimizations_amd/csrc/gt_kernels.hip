@@ -623,29 +623,34 @@ struct DetParams {
 // y[i] = max(x[i .. i+K-1]) for i in [0, N) (van Herk / Gil-Werman): the
 // input is cut into blocks of K; a window is the suffix maximum of one block
 // joined with the prefix maximum of the next.  All indices are compile-time.
+//
+// The detect kernels run it on float bit patterns as signed integers: their
+// values are finite floats >= +0 (bandpass output, never -0) or -inf (outside
+// the frame), for which signed integer order is float order and equal values
+// have equal bits.  Integer max needs no NaN canonicalisation.
 template <int N, int K>
-__device__ __forceinline__ void sliding_max(const float (&x)[N + K - 1], float (&y)[N]) {
+__device__ __forceinline__ void sliding_max(const int32_t (&x)[N + K - 1], int32_t (&y)[N]) {
     constexpr int L = N + K - 1;
-    float S[N];   // suffix max to the end of the block, positions 0..N-1
-    float Pm[L];  // prefix max from the start of the block
+    int32_t S[N];   // suffix max to the end of the block, positions 0..N-1
+    int32_t Pm[L];  // prefix max from the start of the block
 #pragma unroll
     for (int j = N - 1; j >= 0; --j) {
         const int be = ((j / K) + 1) * K - 1 < L - 1 ? ((j / K) + 1) * K - 1 : L - 1;
         if (j == be) {
             S[j] = x[j];
         } else if (j + 1 < N) {
-            S[j] = fmaxf(x[j], S[j + 1]);
+            S[j] = max(x[j], S[j + 1]);
         } else {
-            float m = x[j];
+            int32_t m = x[j];
 #pragma unroll
-            for (int q = j + 1; q <= be; ++q) m = fmaxf(m, x[q]);
+            for (int q = j + 1; q <= be; ++q) m = max(m, x[q]);
             S[j] = m;
         }
     }
 #pragma unroll
-    for (int j = 0; j < L; ++j) Pm[j] = (j % K == 0) ? x[j] : fmaxf(Pm[j - 1], x[j]);
+    for (int j = 0; j < L; ++j) Pm[j] = (j % K == 0) ? x[j] : max(Pm[j - 1], x[j]);
 #pragma unroll
-    for (int i = 0; i < N; ++i) y[i] = fmaxf(S[i], Pm[i + K - 1]);
+    for (int i = 0; i < N; ++i) y[i] = max(S[i], Pm[i + K - 1]);
 }
 
 // Centroid walk of the tile's candidates (centers.wgsl:23-109 and the walk
@@ -821,25 +826,27 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     stage_tile(true);
     __syncthreads();
 
+    // the max filter works on the bit patterns (see sliding_max)
+    const int32_t* Li = (const int32_t*)L;
     // centre values of this thread's 16 outputs, read before the overwrite
     const int c = tid & 63;
     const int kw = __builtin_amdgcn_readfirstlane(tid >> 6);
-    float center[N];
+    int32_t center[N];
 #pragma unroll
-    for (int m = 0; m < N; ++m) center[m] = L[(kw * N + m + a) * LS + (c + a)];
+    for (int m = 0; m < N; ++m) center[m] = Li[(kw * N + m + a) * LS + (c + a)];
 
     // row pass: RM[r][c] = max(L[r][c .. c+KD-1]), written over L[r][0..63]
-    float y[NRUN][N];
+    int32_t y[NRUN][N];
 #pragma unroll
     for (int k2 = 0; k2 < NRUN; ++k2) {
         const int run = tid + k2 * 256;
         if (run < LR * 4) {
             const int r = run >> 2, c0 = (run & 3) * N;
-            float x[NIN];
-            const float* src = L + r * LS + c0;
+            int32_t x[NIN];
+            const int32_t* src = Li + r * LS + c0;
 #pragma unroll
             for (int m = 0; m < NV4; ++m) {
-                const float4 q = *(const float4*)(src + 4 * m);
+                const int4 q = *(const int4*)(src + 4 * m);
                 if (4 * m + 0 < NIN) x[4 * m + 0] = q.x;
                 if (4 * m + 1 < NIN) x[4 * m + 1] = q.y;
                 if (4 * m + 2 < NIN) x[4 * m + 2] = q.z;
@@ -854,35 +861,36 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         const int run = tid + k2 * 256;
         if (run < LR * 4) {
             const int r = run >> 2, c0 = (run & 3) * N;
-            float* dst = L + r * LS + c0;
+            int32_t* dst = (int32_t*)L + r * LS + c0;
 #pragma unroll
             for (int m = 0; m < N / 4; ++m)
-                *(float4*)(dst + 4 * m) =
-                    make_float4(y[k2][4 * m], y[k2][4 * m + 1], y[k2][4 * m + 2], y[k2][4 * m + 3]);
+                *(int4*)(dst + 4 * m) =
+                    make_int4(y[k2][4 * m], y[k2][4 * m + 1], y[k2][4 * m + 2], y[k2][4 * m + 3]);
         }
     }
     __syncthreads();
 
     // column pass + candidate test: lane = column, wave = 16-row run
     const int v = x0 + c;
-    float cm[N];
+    int32_t cm[N];
     {
-        float x[NIN];
+        int32_t x[NIN];
 #pragma unroll
-        for (int m = 0; m < NIN; ++m) x[m] = L[(kw * N + m) * LS + c];
+        for (int m = 0; m < NIN; ++m) x[m] = Li[(kw * N + m) * LS + c];
         sliding_max<N, KD>(x, cm);
     }
 
     // candidates: the ballots of the wave's 16 rows stay in scalar registers
-    const bool colok = (v < W) && (v - dp.band >= 0) && (v + dp.band < W);
+    // (bitwise tests: no short-circuit branches)
+    const bool colok = (v < W) & (v - dp.band >= 0) & (v + dp.band < W);
+    const bool diw = dp.diw != 0;
     uint64_t bal[N];
     uint32_t mycnt = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
         const int u = y0 + kw * N + m;
-        const bool rowok = (u < H) && (u - dp.band >= 0) && (u + dp.band < H);  // uniform
-        const bool cand = colok && rowok && (center[m] >= cm[m]) &&
-                          !(dp.diw && center[m] == 0.f);
+        const bool rowok = (u < H) & (u - dp.band >= 0) & (u + dp.band < H);  // uniform
+        const bool cand = colok & rowok & (center[m] >= cm[m]) & !(diw & (center[m] == 0));
         bal[m] = __ballot(cand);
         if (c == m) mycnt = (uint32_t)__popcll(bal[m]);
     }
@@ -903,7 +911,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         uint32_t mbits = 0;
 #pragma unroll
         for (int m = N - 1; m >= 0; --m) {
-            const uint32_t bits = __float_as_uint(center[m]);
+            const uint32_t bits = (uint32_t)center[m];
             mbits = 2u * mbits + ((bits - base < width) ? 1u : 0u);
         }
         if (__ballot(mbits != 0u)) {  // wave-uniform
@@ -917,7 +925,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
             GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
 #pragma unroll
             for (int m = 0; m < N; ++m) {
-                if (mbits & (1u << m)) list[j++] = __float_as_uint(center[m]);
+                if (mbits & (1u << m)) list[j++] = (uint32_t)center[m];
             }
         }
     }
