@@ -684,10 +684,6 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                 const int br = u - y0 + a - RI, bc = v - x0 + a - RI;
                 float s0 = 0.f, s1 = 0.f, ms = 0.f;
                 if (br >= 0 && br + BW <= LR && bc >= 0 && bc + BW <= LC) {
-                    // Taps with a zero offset add d * 0 = +0 (d >= 0 inside
-                    // the frame) to a sum that is never -0 (it starts at +0,
-                    // and an exact zero result of an fma rounds to +0), which
-                    // leaves it unchanged: those fmas are skipped.
                     const float* box = L + br * LS + bc;
 #pragma unroll
                     for (int di = -RI; di <= RI; ++di)
@@ -695,8 +691,8 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                         for (int dj = -RI; dj <= RI; ++dj) {
                             if (di * di + dj * dj > RI * RI) continue;
                             const float d = box[(di + RI) * LS + (dj + RI)];
-                            if (di != 0) s0 = __builtin_fmaf(d, (float)di, s0);
-                            if (dj != 0) s1 = __builtin_fmaf(d, (float)dj, s1);
+                            s0 = __builtin_fmaf(d, (float)di, s0);
+                            s1 = __builtin_fmaf(d, (float)dj, s1);
                             ms = ms + d;
                         }
                 } else {
