@@ -75,6 +75,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
                     help="locate without the percentile test (the reference's default)")
+    ap.add_argument("--seg-rows", type=int, default=0,
+                    help="rows per bandpass work item (default: the library's 128)")
     ap.add_argument("--separate-refine", action="store_true",
                     help="walk in a separate refine kernel instead of inside detect")
     ap.add_argument("--autotune", action="store_true",
@@ -110,6 +112,8 @@ def main() -> int:
         ctx.set_batch(args.batch)
     if args.separate_refine:
         ctx.set_fused_walk(False)
+    if args.seg_rows:
+        ctx.set_launch_shape(args.seg_rows)
     L = _lib.load()
     import ctypes
 

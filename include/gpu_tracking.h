@@ -151,6 +151,9 @@ gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch);
  * graph holds no memset node.  0 (default): launches are issued directly. */
 gt_status gt_context_set_graphs(gt_context* ctx, int enable);
 
+/* Rows per bandpass work item (16..272, default 128); clears tuned shapes. */
+gt_status gt_context_set_launch_shape(gt_context* ctx, uint32_t seg_rows);
+
 /* Autotuner: times the batched path on n_frames synthetic uint16 frames of
  * h x w for every launch shape (rows per bandpass work item 64/128/256,
  * frames per launch automatic/16/32) and keeps the fastest in the context

@@ -208,3 +208,31 @@ def test_autotune_keeps_results(frames512):
     assert t["seg_rows"] in (64, 128, 256) and t["frames_per_launch"] in (0, 16, 32)
     got = locate(frames512, p, ctx=ctx)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+
+
+def test_uneven_batch_matches_oracle(ctx, frames512):
+    """gt_locate_batch with frames of different shapes in one call: every
+    frame's rows equal the oracle's, and frame columns carry the frame index
+    given in the descriptor."""
+    import ctypes
+    from tracking_optimizations_amd._lib import GtFrameDesc, Result, check, load
+    shapes = [(256, 256), (100, 132), (512, 512), (64, 200), (300, 64), (256, 256)]
+    frames = [np.ascontiguousarray(frames512[i % 3, :h, :w]) for i, (h, w) in enumerate(shapes)]
+    kw = CASES["c1"]
+    descs = (GtFrameDesc * len(frames))()
+    for i, f in enumerate(frames):
+        descs[i].data = f.ctypes.data
+        descs[i].h, descs[i].w = f.shape
+        descs[i].frame_index = 10 + i
+    out = ctypes.c_void_p()
+    p = make_params(**kw)
+    print("[parity] uneven batch", flush=True)
+    check(load().gt_locate_batch(ctx.handle, descs, len(frames), 0, ctypes.byref(p), 0,
+                                 ctypes.byref(out)))
+    r = Result(out)
+    rows, cand = r.rows(), r.candidates()
+    for i, f in enumerate(frames):
+        sel = rows[:, 0] == 10 + i
+        ref_rows, ref_cand, _ = oracle.locate_frame(f.astype(np.float32), _oracle_params(kw),
+                                                    frame_index=10 + i)
+        _check(rows[sel], cand[sel], ref_rows, ref_cand, f"uneven/{i}/{f.shape}")

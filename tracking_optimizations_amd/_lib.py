@@ -34,7 +34,8 @@ EXPORTS = [
     "gt_params_default", "gt_params_rust_default", "gt_params_validate", "gt_column_names",
     "gt_last_error", "gt_version", "gt_device_count", "gt_context_create",
     "gt_context_destroy", "gt_context_timing", "gt_context_set_batch",
-    "gt_context_set_fused_walk", "gt_context_set_graphs", "gt_context_autotune", "gt_locate_u16",
+    "gt_context_set_fused_walk", "gt_context_set_graphs", "gt_context_autotune",
+    "gt_context_set_launch_shape", "gt_locate_u16",
     "gt_locate_u8", "gt_locate_f32", "gt_locate_batch", "gt_result_rows", "gt_result_ncols",
     "gt_result_data", "gt_result_candidates", "gt_result_device_data", "gt_result_columns",
     "gt_result_free", "gt_synth_u16", "gt_device_alloc", "gt_device_free", "gt_memcpy_h2d",
@@ -158,6 +159,7 @@ def load() -> ctypes.CDLL:
         L.gt_context_set_batch.argtypes = [vp, ctypes.c_uint32]
         L.gt_context_set_fused_walk.argtypes = [vp, ctypes.c_int]
         L.gt_context_set_graphs.argtypes = [vp, ctypes.c_int]
+        L.gt_context_set_launch_shape.argtypes = [vp, ctypes.c_uint32]
         L.gt_context_autotune.argtypes = [vp, P(GtParams), ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_uint32, P(GtTuning)]
         for name in ("gt_locate_u16", "gt_locate_u8", "gt_locate_f32"):
@@ -267,6 +269,9 @@ class Context:
 
     def set_fused_walk(self, enable: bool) -> None:
         check(load().gt_context_set_fused_walk(self.handle, 1 if enable else 0))
+
+    def set_launch_shape(self, seg_rows: int) -> None:
+        check(load().gt_context_set_launch_shape(self.handle, seg_rows))
 
     def set_graphs(self, enable: bool) -> None:
         check(load().gt_context_set_graphs(self.handle, 1 if enable else 0))

@@ -1061,6 +1061,15 @@ gt_status gt_context_stage_times(const gt_context* ctx, double* ms, uint64_t* la
     return GT_OK;
 }
 
+gt_status gt_context_set_launch_shape(gt_context* ctx, uint32_t seg_rows) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (seg_rows < 16 || seg_rows > 272)
+        return fail(GT_E_INVALID, "rows per bandpass work item must be in [16, 272]");
+    ctx->seg_rows = (int)seg_rows;
+    ctx->tuned.clear();
+    return GT_OK;
+}
+
 gt_status gt_context_set_graphs(gt_context* ctx, int enable) {
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
     ctx->graphs = enable ? 1 : 0;
