@@ -236,3 +236,17 @@ def test_uneven_batch_matches_oracle(ctx, frames512):
         ref_rows, ref_cand, _ = oracle.locate_frame(f.astype(np.float32), _oracle_params(kw),
                                                     frame_index=10 + i)
         _check(rows[sel], cand[sel], ref_rows, ref_cand, f"uneven/{i}/{f.shape}")
+
+
+@pytest.mark.parametrize("case", ["c1", "c2", "char", "small_sep"])
+@pytest.mark.parametrize("plain", [False, True])
+def test_row_order_matches_reference(ctx, frames512, case, plain):
+    """Rows come in frame order and, within a frame, in raster order of the
+    candidate pixel, as get_work (execute_gpu.rs:108-195) emits them: the
+    unsorted output equals the oracle's row for row."""
+    kw = CASES[case]
+    rows, cand, _, _ = locate(frames512, make_params(**kw), plain=plain, ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(frames512, kw)
+    assert np.array_equal(cand, ref_cand)
+    assert np.array_equal(rows[:, 0], ref_rows[:, 0])
+    assert np.abs(rows[:, 2:4] - ref_rows[:, 2:4]).max() <= POS_TOL

@@ -563,7 +563,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         HIP_TRY(w.rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
         HIP_TRY(w.keep.ensure(slot_cap));
         if (!on_device) HIP_TRY(w.in_stage.ensure(maxneed.in_bytes));
-        HIP_TRY(w.tile3.ensure(al256(maxneed.tiles * 4) * 4));
+        HIP_TRY(w.tile3.ensure(al256(maxneed.tiles * 4) * 4 + 2 * al256(maxneed.tiles * 4 * TILE_H)));
         HIP_TRY(w.perframe.ensure(pf_hist + 3 * pf_small + pf_sel + 2 * pf_8));
     }
     HIP_TRY(ctx->counters.ensure(256));
@@ -688,6 +688,8 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             b.tile_keep = (uint32_t*)(t3 + t3s);
             b.tile_row_off = (uint32_t*)(t3 + 2 * t3s);
             b.tile_lcount = (uint32_t*)(t3 + 3 * t3s);
+            b.rowk = (uint32_t*)(t3 + 4 * t3s);
+            b.rowoff = (uint32_t*)(t3 + 4 * t3s + al256(maxneed.tiles * 4 * TILE_H));
             char* pf = w.perframe.as<char>();
             b.hist1 = g.has_pct ? (uint32_t*)pf : nullptr;
             pf += pf_hist;

@@ -92,6 +92,8 @@ struct Batch {
     uint32_t* tile_count;
     uint32_t* tile_keep;
     uint32_t* tile_row_off;
+    uint32_t* rowk;        // [tiles * 64] kept rows of the tile in pixel rows <= r
+    uint32_t* rowoff;      // [tiles * 64] frame-relative first row of pixel row r
     uint32_t* tile_lcount; // percentile list entries of the tile
     // per-frame arrays [nframes]
     uint32_t* hist1;       // [nframes * HIST_BINS]

@@ -882,7 +882,9 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
 
     // candidates: the ballots of the wave's 16 rows stay in scalar registers
     // (bitwise tests: no short-circuit branches)
-    const bool colok = (v < W) & (v - dp.band >= 0) & (v + dp.band < W);
+    // (lane masks combined in scalar registers: one vector compare each)
+    const uint64_t colmask = __builtin_amdgcn_ballot_w64((v < W) & (v - dp.band >= 0) &
+                                                         (v + dp.band < W));
     const bool diw = dp.diw != 0;
     uint64_t bal[N];
     uint32_t mycnt = 0;
@@ -890,9 +892,10 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     for (int m = 0; m < N; ++m) {
         const int u = y0 + kw * N + m;
         const bool rowok = (u < H) & (u - dp.band >= 0) & (u + dp.band < H);  // uniform
-        const bool cand = colok & rowok & (center[m] >= cm[m]) & !(diw & (center[m] == 0));
-        bal[m] = __ballot(cand);
-        if (c == m) mycnt = (uint32_t)__popcll(bal[m]);
+        const uint64_t ge = __builtin_amdgcn_ballot_w64(center[m] >= cm[m]);
+        const uint64_t nz = diw ? __builtin_amdgcn_ballot_w64(center[m] != 0) : ~0ull;
+        bal[m] = rowok ? (colmask & ge & nz) : 0ull;
+        mycnt = (c == m) ? (uint32_t)__popcll(bal[m]) : mycnt;
     }
     if (c < N) s_rowcnt[kw * N + c] = mycnt;
 
@@ -944,6 +947,7 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
+        if (bal[m] == 0) continue;  // wave-uniform: most 64-pixel row pieces hold no candidate
         const int r = kw * N + m;
         if ((bal[m] >> c) & 1ull) {
             const uint32_t rank =
@@ -1689,9 +1693,11 @@ constexpr int DD_CAP = 256;   // staged records per wave (more fall back to glob
 __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__ g) {
     __shared__ float s_m[4][DD_CAP], s_y[4][DD_CAP], s_x[4][DD_CAP];
     __shared__ int32_t s_i[4][DD_CAP];
+    __shared__ uint32_t s_kr[4][TILE_H];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
     if (item >= b.tiles) return;
+    s_kr[wave][lane] = 0;
     const uint64_t cap = b.slot_cap;
     const GLOBAL float* rmass = (const GLOBAL float*)(b.rec + REC_MASS * cap);
     const GLOBAL float* ry = (const GLOBAL float*)(b.rec + REC_Y * cap);
@@ -1740,6 +1746,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     if (!(bymin <= bymax)) {  // no valid own record
         for (uint32_t i = lane; i < cnt; i += 64) b.keep[slot0 + i] = 0;
         if (lane == 0) b.tile_keep[item] = 0;
+        b.rowk[(size_t)item * TILE_H + lane] = 0;
         return;
     }
     const float margin = rad + 1.0f;
@@ -1862,9 +1869,15 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
             }
         }
         if (own) b.keep[slot0 + i] = keep ? 1 : 0;
+        if (keep) atomicAdd(&s_kr[wave][(uint32_t)(idx / W) - (uint32_t)(ty * TILE_H)], 1u);
         kept += (uint32_t)__popcll(__ballot(keep));
     }
     if (lane == 0) b.tile_keep[item] = kept;
+    // kept rows of this tile in pixel rows <= r (the emit order is raster)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    b.rowk[(size_t)item * TILE_H + lane] = wave_incl_scan(s_kr[wave][lane], lane);
 }
 
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
@@ -1904,6 +1917,29 @@ __global__ __launch_bounds__(256) void k_tile_scan(Batch b) {
     if (t == 0) {
         b.frame_rows[f] = carry;
         b.frame_cands[f] = s_cand[0] + s_cand[1] + s_cand[2] + s_cand[3];
+    }
+    // Raster order within the frame (get_work scans masses in raster order,
+    // execute_gpu.rs:108-195): a kept row at pixel row r of tile t in tile
+    // row tr comes after every kept row of the earlier tile rows, of tiles
+    // left of t in pixel rows <= r and of tiles right of t in pixel rows < r;
+    // the rows of tile t itself are in raster order already.  With
+    // K[t][r] = kept rows of tile t in pixel rows <= r (from dedup):
+    //   rowoff[t][r] = base(tr) + sum_{t' < t} K[t'][r] + sum_{t' > t} K[t'][r - 1].
+    // Wave w handles tile rows w, w + 4, ...; lane = pixel row.
+    for (uint32_t tr = (uint32_t)wave; tr < fr.ty; tr += 4) {
+        const uint32_t t0 = fr.tile_base + tr * fr.tx;
+        const uint32_t base = b.tile_row_off[t0];
+        uint32_t total_lt = 0;
+        for (uint32_t q = 0; q < fr.tx; ++q)
+            total_lt += lane > 0 ? b.rowk[(size_t)(t0 + q) * TILE_H + lane - 1] : 0u;
+        uint32_t pre_le = 0, cum_lt = 0;
+        for (uint32_t q = 0; q < fr.tx; ++q) {
+            const size_t o = (size_t)(t0 + q) * TILE_H + lane;
+            const uint32_t kle = b.rowk[o];
+            cum_lt += lane > 0 ? b.rowk[o - 1] : 0u;
+            b.rowoff[o] = base + pre_le + (total_lt - cum_lt);
+            pre_le += kle;
+        }
     }
 }
 
@@ -1957,7 +1993,8 @@ hipError_t launch_frame_scan(const Batch& b, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // K5 emit: rows [frame, mass, y, x, (Rg, ecc, signal), (Rg_gpu, raw_mass_gpu,
-// signal_gpu, ecc_gpu)] in frame, tile, raster-within-tile order.
+// signal_gpu, ecc_gpu)] in frame order, raster order of the candidate pixel
+// within a frame (row positions from k_tile_scan).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1972,13 +2009,16 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
     const size_t slot0 = (size_t)item * TILE_PX;
     const uint64_t cap = b.slot_cap;
     const int nc = g->ncols;
-    uint64_t row = b.frame_row_base[f] + b.tile_row_off[item];
+    const uint64_t fbase = b.frame_row_base[f];
+    uint32_t before = 0;  // kept rows of this tile before the chunk
     for (uint32_t ib = 0; ib < cnt; ib += 64) {
         const uint32_t i = ib + lane;
         const bool k = (i < cnt) && b.keep[slot0 + i];
         const uint64_t msk = __ballot(k);
         if (k) {
-            const uint64_t r = row + (uint64_t)__popcll(msk & lanemask_lt(lane));
+            const uint32_t l = b.slots[slot0 + i];
+            const uint64_t r = fbase + b.rowoff[(size_t)item * TILE_H + l / TILE_W] + before +
+                               (uint64_t)__popcll(msk & lanemask_lt(lane));
             if (r < b.rows_cap) {
                 float* o = b.rows + r * (uint64_t)nc;
                 const size_t s = slot0 + i;
@@ -1998,12 +2038,11 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
                     o[c++] = b.rec[REC_SIG_GPU * cap + s];
                     o[c++] = b.rec[REC_ECC_GPU * cap + s];
                 }
-                const uint32_t l = b.slots[s];
                 b.rows_cand[r] = (int64_t)(ty * TILE_H + (int)(l / TILE_W)) * W +
                                  (tx * TILE_W + (int)(l % TILE_W));
             }
         }
-        row += (uint64_t)__popcll(msk);
+        before += (uint32_t)__popcll(msk);
     }
 }
 
