@@ -746,7 +746,10 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             }
             HIP_TRY(stage(ctx, st, GT_STAGE_REFINE, [&] { return launch_refine(b, g, d_geom, st); }));
             HIP_TRY(stage(ctx, st, GT_STAGE_DEDUP, [&] { return launch_dedup(b, g, d_geom, st); }));
-            HIP_TRY(stage(ctx, st, GT_STAGE_SCAN, [&] { return launch_tile_scan(b, st); }));
+            uint32_t max_ty = 1;
+            for (size_t fi = bp.b0; fi < bp.b1; ++fi)
+                max_ty = std::max(max_ty, ftab[fi].ty);
+            HIP_TRY(stage(ctx, st, GT_STAGE_SCAN, [&] { return launch_tile_scan(b, max_ty, st); }));
             // row bases continue the previous batch's count: order the scans
             if (nstreams > 1 && bi > 0)
                 HIP_TRY(hipStreamWaitEvent(st, ctx->ev_scan[(bi - 1) % nstreams], 0));

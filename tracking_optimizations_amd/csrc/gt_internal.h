@@ -143,7 +143,7 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom,
                         hipStream_t s);
-hipError_t launch_tile_scan(const Batch& b, hipStream_t s);
+hipError_t launch_tile_scan(const Batch& b, uint32_t max_ty, hipStream_t s);
 hipError_t launch_frame_scan(const Batch& b, hipStream_t s);
 hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
                        hipStream_t s);

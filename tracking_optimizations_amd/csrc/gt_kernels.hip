@@ -1918,33 +1918,61 @@ __global__ __launch_bounds__(256) void k_tile_scan(Batch b) {
         b.frame_rows[f] = carry;
         b.frame_cands[f] = s_cand[0] + s_cand[1] + s_cand[2] + s_cand[3];
     }
-    // Raster order within the frame (get_work scans masses in raster order,
-    // execute_gpu.rs:108-195): a kept row at pixel row r of tile t in tile
-    // row tr comes after every kept row of the earlier tile rows, of tiles
-    // left of t in pixel rows <= r and of tiles right of t in pixel rows < r;
-    // the rows of tile t itself are in raster order already.  With
-    // K[t][r] = kept rows of tile t in pixel rows <= r (from dedup):
-    //   rowoff[t][r] = base(tr) + sum_{t' < t} K[t'][r] + sum_{t' > t} K[t'][r - 1].
-    // Wave w handles tile rows w, w + 4, ...; lane = pixel row.
-    for (uint32_t tr = (uint32_t)wave; tr < fr.ty; tr += 4) {
-        const uint32_t t0 = fr.tile_base + tr * fr.tx;
-        const uint32_t base = b.tile_row_off[t0];
-        uint32_t total_lt = 0;
-        for (uint32_t q = 0; q < fr.tx; ++q)
-            total_lt += lane > 0 ? b.rowk[(size_t)(t0 + q) * TILE_H + lane - 1] : 0u;
-        uint32_t pre_le = 0, cum_lt = 0;
-        for (uint32_t q = 0; q < fr.tx; ++q) {
-            const size_t o = (size_t)(t0 + q) * TILE_H + lane;
-            const uint32_t kle = b.rowk[o];
-            cum_lt += lane > 0 ? b.rowk[o - 1] : 0u;
-            b.rowoff[o] = base + pre_le + (total_lt - cum_lt);
-            pre_le += kle;
+}
+
+// Raster order within a frame (get_work scans masses in raster order,
+// execute_gpu.rs:108-195): a kept row at pixel row r of tile t in tile row tr
+// comes after every kept row of the earlier tile rows, of tiles left of t in
+// pixel rows <= r and of tiles right of t in pixel rows < r; the rows of tile
+// t itself are in raster order already.  With K[t][r] = kept rows of tile t
+// in pixel rows <= r (from dedup):
+//   rowoff[t][r] = base(tr) + sum_{t' < t} K[t'][r] + sum_{t' > t} K[t'][r - 1].
+// One wave per tile row, lane = pixel row; the loads of 32 tiles go out
+// together.
+__global__ __launch_bounds__(256) void k_row_scan(Batch b) {
+    constexpr int CH = 32;
+    const uint32_t f = blockIdx.x;
+    const Frame fr = b.frames[f];
+    const int lane = threadIdx.x & 63;
+    const uint32_t tr = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (tr >= fr.ty) return;
+    const uint32_t t0 = fr.tile_base + tr * fr.tx;
+    const uint32_t base = b.tile_row_off[t0];
+    const uint32_t* K = b.rowk + (size_t)t0 * TILE_H + lane;
+    uint32_t total_lt = 0;
+    for (uint32_t q0 = 0; q0 < fr.tx; q0 += CH) {
+        uint32_t v[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j)
+            v[j] = (q0 + j < fr.tx && lane > 0) ? K[(size_t)(q0 + j) * TILE_H - 1] : 0u;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) total_lt += v[j];
+    }
+    uint32_t pre_le = 0, cum_lt = 0;
+    for (uint32_t q0 = 0; q0 < fr.tx; q0 += CH) {
+        uint32_t kle[CH], klt[CH];
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const bool in = q0 + j < fr.tx;
+            kle[j] = in ? K[(size_t)(q0 + j) * TILE_H] : 0u;
+            klt[j] = (in && lane > 0) ? K[(size_t)(q0 + j) * TILE_H - 1] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            if (q0 + j < fr.tx) {
+                cum_lt += klt[j];
+                b.rowoff[(size_t)(t0 + q0 + j) * TILE_H + lane] = base + pre_le + (total_lt - cum_lt);
+                pre_le += kle[j];
+            }
         }
     }
 }
 
-hipError_t launch_tile_scan(const Batch& b, hipStream_t s) {
+hipError_t launch_tile_scan(const Batch& b, uint32_t max_ty, hipStream_t s) {
     hipLaunchKernelGGL(k_tile_scan, dim3(b.nframes), dim3(256), 0, s, b);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_row_scan, dim3(b.nframes, (max_ty + 3) / 4), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
