@@ -85,9 +85,12 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
-    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint"], default="movie",
+    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link"], default="movie",
                     help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames; "
-                         "adjoint: 256 x 512^2 localisation errors and Jacobians")
+                         "adjoint: 256 x 512^2 localisation errors and Jacobians; "
+                         "link: locate + link, 2000 x 1024^2, 3000 blinking particles")
+    ap.add_argument("--link-cpu-frames", type=int, default=150,
+                    help="frames of the link workload timed with the CPU oracle linker")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +105,8 @@ def main() -> int:
         return bench_uneven(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "link":
+        return bench_link(args)
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
     if args.no_percentile:
@@ -419,6 +424,101 @@ def bench_adjoint(args) -> int:
                                  "params": kw, "rows": res["errors"][1]},
                       "with_jacobian_frames_per_s": res["jacobian"][0],
                       "forward_only_frames_per_s": fwd}), flush=True)
+    movie.free()
+    return 0
+
+
+def bench_link(args) -> int:
+    """Locate + link: 2000 frames of 1024x1024 uint16, 3000 Brownian particles
+    (D = 0.5 px^2/frame, sigma 1.8, lognormal peaks with median 1200,
+    background 120 + Poisson), each off in a frame with probability 0.05;
+    diameter 9, separation 9, minmass set from frame 0 (a quarter of the
+    median mass of its brightest rows, as many rows as spots drawn),
+    search_range 5, memory 3.  A step is one locate call with linking (rows
+    with the particle column back on the host).  Beside it: locate alone,
+    gt_link alone on the same row table, and the CPU oracle linker on the
+    first --link-cpu-frames frames."""
+    import ctypes
+    from oracle.linker import link_rows
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import link_rust, make_params
+    ctx = _lib.Context(0)
+    L = _lib.load()
+    nf = 2000 if args.frames == 1000 else args.frames
+    H = W = 1024
+    log(f"generating {nf} frames of {H}x{W} uint16 (3000 blinking particles)")
+    spec = synth.link_1024(seed=5, n_frames=nf)
+    movie = synth.generate(ctx, spec)
+    ctx.synchronize()
+
+    def locate_rows(params, n):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), n, H, W,
+                                   ctypes.byref(params), _lib.GT_FRAMES_ON_DEVICE,
+                                   ctypes.byref(out)))
+        return _lib.Result(out).rows()
+
+    kw = dict(diameter=9, separation=9)
+    r0 = locate_rows(make_params(**kw, minmass=0.0), 1)
+    n0 = int(spec.spot_off[1])
+    top = np.sort(r0[:, 1])[::-1][:n0]
+    kw["minmass"] = float(0.25 * np.median(top))
+    log(f"minmass from frame 0: {kw['minmass']:.1f}")
+    p_locate = make_params(**kw)
+    p_link = make_params(**kw, search_range=5.0, memory=3)
+
+    for _ in range(args.warmup):
+        locate_rows(p_link, nf)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rows = locate_rows(p_link, nf)
+    el = time.perf_counter() - t0
+    st = ctx.link_stats()
+    fps = nf * args.steps / el
+    log(f"locate + link: {fps:.0f} frames/s, {rows.shape[0]} rows, link search "
+        f"{st['search_ms']:.1f} ms, solve {st['solve_ms']:.1f} ms, {st['subnets']} subnets "
+        f"(largest {st['max_subnet']})")
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plain = locate_rows(p_locate, nf)
+    loc_fps = nf * args.steps / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ids = link_rust(plain, 5.0, 3, ctx=ctx)
+    link_el = (time.perf_counter() - t0) / args.steps
+    lst = ctx.link_stats()
+    log(f"locate alone {loc_fps:.0f} frames/s; link_rust alone {nf / link_el:.0f} frames/s "
+        f"({link_el * 1e3:.0f} ms: search {lst['search_ms']:.1f}, solve {lst['solve_ms']:.1f})")
+
+    cpu = None
+    if not args.no_cpu:
+        k = min(args.link_cpu_frames, nf)
+        sub = plain[plain[:, 0] < k]
+        t0 = time.perf_counter()
+        ref = link_rows(sub, 5.0, 3)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(ref, ids[: len(ref)]))
+        cpu = {"value": k / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"oracle linker (Python/numpy) on the first {k} frames' rows "
+                         f"({len(sub)} rows), link only; ids identical to the GPU linker: "
+                         f"{same}"}
+        log(f"CPU oracle linker: {k / dt:.1f} frames/s on {k} frames; ids identical: {same}")
+    print(json.dumps({"metric": "frames/s, locate + link (search_range 5, memory 3), "
+                                "2000 x 1024^2 uint16, 3000 particles",
+                      "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": 1000 * el / args.steps,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32",
+                      "data": "synthetic (seeded device generator), device-resident frames, "
+                              "rows returned to the host",
+                      "config": {"workload": f"{nf} x {H}x{W} uint16, 3000 Brownian particles, "
+                                             "blink-off p 0.05", "params": kw,
+                                 "search_range": 5.0, "memory": 3, "rows": int(rows.shape[0])},
+                      "link": {k2: st[k2] for k2 in st},
+                      "locate_only_frames_per_s": loc_fps,
+                      "link_rust_only_frames_per_s": nf / link_el,
+                      "cpu_baseline": cpu}), flush=True)
     movie.free()
     return 0
 

@@ -270,6 +270,32 @@ gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad
                          float* grad_frames, uint32_t flags);
 void gt_adjoint_free(gt_adjoint* a);
 
+/* --- linking (gpu-tracking/src/linking.rs) --------------------------------- */
+
+/* Statistics of the last link on a context. */
+typedef struct gt_link_stats {
+    uint64_t steps;         /* Linker::advance calls */
+    uint64_t pairs;         /* (source, destination) pairs within the radius, all ages */
+    uint64_t subnets;       /* subnets solved by exhaustive assignment */
+    uint64_t max_subnet;    /* destinations of the largest subnet */
+    uint64_t ids;           /* particle ids handed out */
+    double search_ms;       /* host wall time of the GPU candidate search */
+    double solve_ms;        /* host wall time of the subnet solver */
+} gt_link_stats;
+
+/* link_rust (gpu-tracking/src/lib.rs:202-212 -> FrameSubsetter, linking.rs:
+ * 12-61 -> linker_all, linking.rs:570-580): ids[i] = particle id of row i of
+ * a row-major n_rows x ncols float32 table in host memory, column 0 = frame,
+ * columns 2 and 3 = y and x.  A step is a run of equal frame values (a first
+ * frame other than 0 gives an empty first step); search_range and memory as
+ * Linker::new (linking.rs:211).  Candidates within the radius are found on
+ * the device, subnets are solved on the host. */
+gt_status gt_link(gt_context* ctx, const float* rows, uint64_t n_rows, uint32_t ncols,
+                  float search_range, uint64_t memory, uint64_t* ids);
+
+/* Statistics of the last gt_link or locate call with search_range. */
+gt_status gt_context_link_stats(const gt_context* ctx, gt_link_stats* out);
+
 /* --- synthetic movies (device side, for tests and benchmarks) ------------ */
 
 /* Fills n_frames x h x w uint16 device frames with

@@ -92,7 +92,7 @@ def test_rust_defaults(lib):
 
 @pytest.mark.parametrize("kw,status", [
     (dict(maxsize=3.0), 2), (dict(threshold=0.1), 2), (dict(invert=True), 2),
-    (dict(topn=5), 2), (dict(preprocess=False), 2), (dict(search_range=5.0), 2),
+    (dict(topn=5), 2), (dict(preprocess=False), 2), (dict(search_range=5.0), 0),
     (dict(sig_radius=3.0), 2), (dict(bg_radius=8.0), 2),
     (dict(noise_size=0.0), 1), (dict(percentile=101.0), 1), (dict(percentile=-1.0), 1),
     (dict(smoothing_size=0), 1), (dict(percentile=64.0), 0), (dict(), 0),
@@ -141,13 +141,24 @@ def test_no_device_behaviour(lib):
         _lib.GT_E_INVALID
 
 
+def test_link_argument_checks(lib):
+    """gt_link rejects a missing context before touching a device."""
+    from tracking_optimizations_amd import _lib
+    ids = (ctypes.c_uint64 * 1)()
+    rows = (ctypes.c_float * 4)()
+    assert lib.gt_link(None, rows, 1, 4, 5.0, 0, ids) == _lib.GT_E_INVALID
+    assert "context" in lib.gt_last_error().decode()
+    st = _lib.GtLinkStats()
+    assert lib.gt_context_link_stats(None, ctypes.byref(st)) == _lib.GT_E_INVALID
+
+
 def test_python_api_errors_before_device():
     from tracking_optimizations_amd import GtInvalid, GtNotImplemented, batch
     v = np.zeros((2, 32, 32), np.float32)
     with pytest.raises(GtNotImplemented):
         batch(v, 9, maxsize=3)
     with pytest.raises(GtNotImplemented):
-        batch(v, 9, search_range=4.0)
+        batch(v, 9, sig_radius=4.0)
     with pytest.raises(GtInvalid):
         batch(np.zeros((2, 32, 64), np.float32)[:, :, ::2], 9)
     with pytest.raises(GtInvalid):

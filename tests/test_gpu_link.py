@@ -1,0 +1,152 @@
+"""GPU linker (gt_link, locate with search_range) vs the CPU linker oracle:
+particle ids must be identical, row for row.
+
+Both take a source's candidate destinations in ascending destination order,
+so ids agree exactly; the reference's kd-tree order can differ from it only
+between assignments of equal binary32 cost.  At the full particle count of
+the linking workload (3000 per frame) the check is the size-independent one:
+well-separated tracks are recovered exactly.
+"""
+import numpy as np
+import pytest
+
+from oracle.linker import Linker, link_rows
+from tracking_optimizations_amd import _lib, synth
+from tracking_optimizations_amd.gpu_tracking import batch_rust, link_rust
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return _lib.default_context(0)
+
+
+def brownian_rows(seed, n_frames, n, size, step_sd, p_off, frame0=0, extra_cols=1):
+    """[frame, mass, y, x, extra...] float32 rows of blinking Brownian tracks,
+    particles of a frame in random order."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, size, (n, 2))
+    out = []
+    for f in range(n_frames):
+        on = np.flatnonzero(rng.random(n) >= p_off)
+        on = rng.permutation(on)
+        for k in on:
+            out.append([frame0 + f, rng.uniform(100, 1000), pos[k, 0], pos[k, 1]] +
+                       [rng.normal()] * extra_cols)
+        pos = pos + rng.normal(0, step_sd, pos.shape)
+    return np.asarray(out, np.float32)
+
+
+@pytest.mark.parametrize("memory", [0, 1, 3])
+def test_link_rust_matches_oracle(ctx, memory):
+    rows = brownian_rows(1 + memory, 50, 150, 200.0, 1.0, 0.1)
+    ids = link_rust(rows, 5.0, memory, ctx=ctx)
+    ref = link_rows(rows, 5.0, memory)
+    assert ids.dtype == np.uint64
+    assert np.array_equal(ids, ref)
+    st = ctx.link_stats()
+    assert st["steps"] == 50 and st["ids"] == int(ref.max()) + 1
+
+
+def test_link_rust_dense_subnets(ctx):
+    # about 0.5 other particles within the radius: multi-particle subnets
+    rows = brownian_rows(9, 30, 60, 60.0, 0.7, 0.05)
+    ids = link_rust(rows, 3.0, 2, ctx=ctx)
+    assert np.array_equal(ids, link_rows(rows, 3.0, 2))
+    assert ctx.link_stats()["subnets"] > 0
+
+
+@pytest.mark.parametrize("case", ["empty", "one_row", "first_frame_not_0", "repeated_frames",
+                                  "frame_gaps", "single_step", "four_columns"])
+def test_link_rust_edge_cases(ctx, case):
+    if case == "empty":
+        rows = np.zeros((0, 4), np.float32)
+    elif case == "one_row":
+        rows = np.asarray([[0, 1, 5, 5]], np.float32)
+    elif case == "first_frame_not_0":
+        rows = brownian_rows(3, 10, 20, 50.0, 1.0, 0.1, frame0=3)
+    elif case == "repeated_frames":
+        a = brownian_rows(4, 6, 20, 50.0, 1.0, 0.1)
+        rows = np.concatenate([a, a[a[:, 0] < 2]])   # frame values 0..5, then 0, 1 again
+    elif case == "frame_gaps":
+        a = brownian_rows(5, 12, 20, 50.0, 1.0, 0.1)
+        rows = a[np.isin(a[:, 0], [0, 1, 4, 5, 9, 11])]
+    elif case == "single_step":
+        rows = brownian_rows(6, 1, 30, 50.0, 1.0, 0.0)
+    else:
+        rows = brownian_rows(7, 8, 30, 50.0, 1.0, 0.1, extra_cols=0)
+    for memory in (0, 2):
+        ids = link_rust(rows, 4.0, memory, ctx=ctx)
+        assert np.array_equal(ids, link_rows(rows, 4.0, memory)), (case, memory)
+
+
+def test_link_rust_rejects_narrow_tables(ctx):
+    with pytest.raises(_lib.GtInvalid):
+        link_rust(np.zeros((3, 3), np.float32), 4.0, ctx=ctx)
+
+
+def _oracle_per_frame(rows, n_frames, r, memory):
+    """Linker::advance once per frame, empty frames included
+    (execute_gpu.rs:335)."""
+    lk = Linker(r, memory)
+    out = []
+    for f in range(n_frames):
+        sel = rows[rows[:, 0] == f]
+        out.extend(lk.advance([(np.float32(y), np.float32(x)) for y, x in sel[:, 2:4]]))
+    return np.asarray(out, np.uint64)
+
+
+@pytest.mark.parametrize("memory", [0, 3])
+def test_locate_with_search_range(ctx, memory):
+    spec = synth.link_1024(seed=15, n_frames=24, n_spots=150, size=256)
+    video = synth.generate_host(ctx, spec)
+    # two blank frames in the middle: no particles, but the steps still advance
+    video[10:12] = 0
+    kw = dict(minmass=300.0, separation=9)
+    plain_rows, _cols = batch_rust(video, 9, **kw)
+    rows, cols = batch_rust(video, 9, search_range=5.0, memory=memory, **kw)
+    names = [c[0] for c in cols]
+    assert names[:5] == ["frame", "mass", "y", "x", "particle"]
+    assert rows.shape == (plain_rows.shape[0], plain_rows.shape[1] + 1)
+    assert np.array_equal(rows[:, :4], plain_rows[:, :4])
+    assert np.array_equal(rows[:, 5:], plain_rows[:, 4:])
+    ref = _oracle_per_frame(plain_rows, video.shape[0], 5.0, memory)
+    assert np.array_equal(rows[:, 4], ref.astype(np.float32))
+    assert not np.any(plain_rows[:, 0] == 10) and not np.any(plain_rows[:, 0] == 11)
+
+
+def test_full_size_separated_tracks(ctx):
+    """3000 particles per frame on a jittered 18 px grid, steps below 0.5 px
+    per axis, 5 % blinking: every gap up to `memory` frames is bridged and
+    every longer one starts a new id."""
+    rng = np.random.default_rng(21)
+    g = np.stack(np.meshgrid(np.arange(55), np.arange(55), indexing="ij"), -1).reshape(-1, 2)
+    base = g[:3000] * 18.0 + 20.0 + rng.uniform(-1, 1, (3000, 2))
+    n_frames, memory, r = 300, 3, 5.0
+    rows, truth = [], []
+    pos = base.copy()
+    for f in range(n_frames):
+        on = rng.permutation(np.flatnonzero(rng.random(3000) >= 0.05))
+        rows.append(np.column_stack([np.full(len(on), f), np.ones(len(on)), pos[on]]))
+        truth.append(on)
+        pos = pos + rng.uniform(-0.1, 0.1, pos.shape)
+    rows = np.concatenate(rows).astype(np.float32)
+    ids = link_rust(rows, r, memory, ctx=ctx)
+    last, label, nxt, exp = {}, {}, 0, []
+    for f, ks in enumerate(truth):
+        for k in ks:
+            if k not in last or f - last[k] - 1 > memory:
+                label[k] = nxt
+                nxt += 1
+            exp.append(label[k])
+            last[k] = f
+    exp = np.asarray(exp)
+
+    def relabel(a):
+        m = {}
+        return np.asarray([m.setdefault(int(i), len(m)) for i in a])
+
+    assert np.array_equal(relabel(ids), relabel(exp))
+    st = ctx.link_stats()
+    assert st["steps"] == n_frames and st["max_subnet"] <= 1

@@ -42,7 +42,8 @@ EXPORTS = [
     "gt_memcpy_d2h", "gt_memcpy_d2d", "gt_synchronize", "gt_locate_adjoint", "gt_adjoint_rows",
     "gt_adjoint_ncols", "gt_adjoint_patch", "gt_adjoint_data", "gt_adjoint_candidates",
     "gt_adjoint_sigma", "gt_adjoint_origin", "gt_adjoint_jacobian", "gt_adjoint_vjp",
-    "gt_adjoint_free", "gt_context_set_profiling", "gt_context_stage_times",
+    "gt_adjoint_free", "gt_context_set_profiling", "gt_context_stage_times", "gt_link",
+    "gt_context_link_stats",
 ]
 
 STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
@@ -118,6 +119,18 @@ class GtTuning(ctypes.Structure):
                 ("device_ms_per_frame", ctypes.c_double)]
 
 
+class GtLinkStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_uint64),
+        ("pairs", ctypes.c_uint64),
+        ("subnets", ctypes.c_uint64),
+        ("max_subnet", ctypes.c_uint64),
+        ("ids", ctypes.c_uint64),
+        ("search_ms", ctypes.c_double),
+        ("solve_ms", ctypes.c_double),
+    ]
+
+
 class GtFrameDesc(ctypes.Structure):
     _fields_ = [
         ("data", ctypes.c_void_p),
@@ -191,6 +204,9 @@ def load() -> ctypes.CDLL:
         L.gt_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_memcpy_d2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
         L.gt_synchronize.argtypes = [vp]
+        L.gt_link.argtypes = [vp, P(ctypes.c_float), ctypes.c_uint64, ctypes.c_uint32,
+                              ctypes.c_float, ctypes.c_uint64, P(ctypes.c_uint64)]
+        L.gt_context_link_stats.argtypes = [vp, P(GtLinkStats)]
         L.gt_locate_adjoint.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
                                         ctypes.c_uint32, P(GtParams), ctypes.c_uint32, P(vp)]
         L.gt_adjoint_rows.argtypes = [vp]
@@ -285,6 +301,12 @@ class Context:
 
     def synchronize(self) -> None:
         check(load().gt_synchronize(self.handle))
+
+    def link_stats(self) -> dict:
+        """Statistics of the last link (gt_link or locate with search_range)."""
+        t = GtLinkStats()
+        check(load().gt_context_link_stats(self.handle, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in GtLinkStats._fields_}
 
     def set_profiling(self, enable: bool) -> None:
         check(load().gt_context_set_profiling(self.handle, int(bool(enable))))

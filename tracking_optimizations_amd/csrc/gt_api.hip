@@ -133,8 +133,6 @@ gt_status check_params(const gt_params* p, uint32_t h, uint32_t w) {
     if (p->has_invert) return fail(GT_E_NOT_IMPLEMENTED, "invert is not implemented");
     if (p->has_topn) return fail(GT_E_NOT_IMPLEMENTED, "topn is not implemented");
     if (p->has_preprocess) return fail(GT_E_NOT_IMPLEMENTED, "preprocess is not implemented");
-    if (p->has_search_range)
-        return fail(GT_E_NOT_IMPLEMENTED, "search_range (linking) is not implemented in this build");
     if (p->has_sig_radius || p->has_bg_radius)
         return fail(GT_E_NOT_IMPLEMENTED,
                     "sig_radius/bg_radius (raw signal extraction) is not implemented in this build");
@@ -291,6 +289,7 @@ struct gt_context {
     uint64_t stage_launches[GT_NSTAGES] = {0};
     DevBuf geom, frames_tab, bases, rows, rows_cand, counters;
     Workspace ws[kStreams];
+    LinkStats last_link;
 };
 
 struct gt_result {
@@ -438,6 +437,46 @@ struct AdjReq {
     }
 };
 
+// Linking inside locate (execute_gpu.rs:335, 395-396): Linker::advance runs
+// once per frame, frames without particles included, and every row gets the
+// particle id as a float column right after x (execute_gpu.rs:351).
+gt_status link_result(gt_context* ctx, const std::vector<FrameIn>& frames, const gt_params* p,
+                      gt_result* res) {
+    const uint32_t nc = res->ncols;
+    const uint64_t n = res->rows;
+    std::vector<uint32_t> step_off(frames.size() + 1, 0);
+    std::vector<float> ys(n), xs(n);
+    uint64_t r = 0;
+    for (size_t f = 0; f < frames.size(); ++f) {
+        const float fv = (float)frames[f].fi;
+        for (; r < n && res->data[r * nc] == fv; ++r) {
+            ys[r] = res->data[r * nc + 2];
+            xs[r] = res->data[r * nc + 3];
+        }
+        step_off[f + 1] = (uint32_t)r;
+    }
+    if (r != n) return fail(GT_E_INTERNAL, "link: rows are not in frame order");
+    std::vector<uint64_t> ids(n);
+    const uint32_t mem = p->has_memory ? (uint32_t)std::min<uint64_t>(p->memory, UINT32_MAX) : 0;
+    std::string err;
+    const hipError_t e = link_steps(ctx->streams[0], ys.data(), xs.data(), step_off,
+                                    p->search_range, mem, ids.data(), &ctx->last_link, err);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? GT_E_OOM
+                    : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP, "%s", err.c_str());
+    std::vector<float> out(n * (nc + 1));
+    for (uint64_t i = 0; i < n; ++i) {
+        const float* src = res->data.data() + i * nc;
+        float* dst = out.data() + i * (nc + 1);
+        for (uint32_t c = 0; c < 4; ++c) dst[c] = src[c];
+        dst[4] = (float)ids[i];
+        for (uint32_t c = 4; c < nc; ++c) dst[c + 1] = src[c];
+    }
+    res->data.swap(out);
+    res->ncols = nc + 1;
+    return GT_OK;
+}
+
 gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
                      const gt_params* p, uint32_t flags, gt_result** out,
                      AdjReq* adj = nullptr) {
@@ -472,6 +511,14 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                 batch_frames = it->second.second;
             }
         }
+    }
+    if (p->has_search_range) {
+        if (result_on_device || adj)
+            return fail(GT_E_NOT_IMPLEMENTED,
+                        "search_range needs host results (no GT_RESULT_ON_DEVICE, no adjoint)");
+        for (size_t f = 1; f < frames.size(); ++f)
+            if ((float)frames[f].fi == (float)frames[f - 1].fi)
+                return fail(GT_E_INVALID, "search_range needs consecutive frames with distinct indices");
     }
     if (adj && !adjoint_supported(g))
         return fail(GT_E_NOT_IMPLEMENTED, "adjoint needs (2*rint+1)^2 <= 1024 and smoothing_size/2 <= 16");
@@ -882,6 +929,13 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             }
         }
         break;
+    }
+    if (p->has_search_range) {
+        const gt_status ls = link_result(ctx, frames, p, res);
+        if (ls != GT_OK) {
+            delete res;
+            return ls;
+        }
     }
     ctx->last.rows = res->rows;
     ctx->last.launches = launches;
@@ -1459,6 +1513,35 @@ gt_status gt_memcpy_h2d(gt_context* ctx, void* dst, const void* src, uint64_t by
 gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t bytes) {
     if (!ctx) return fail(GT_E_INVALID, "NULL argument");
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+gt_status gt_link(gt_context* ctx, const float* rows, uint64_t n_rows, uint32_t ncols,
+                  float search_range, uint64_t memory, uint64_t* ids) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (n_rows && (!rows || !ids)) return fail(GT_E_INVALID, "NULL argument");
+    if (ncols < 4) return fail(GT_E_INVALID, "rows need at least 4 columns (frame, mass, y, x)");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::string err;
+    const hipError_t e = link_rows(ctx->streams[0], rows, n_rows, ncols, search_range,
+                                   (uint32_t)std::min<uint64_t>(memory, UINT32_MAX), ids,
+                                   &ctx->last_link, err);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? GT_E_OOM
+                    : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP, "%s", err.c_str());
+    return GT_OK;
+}
+
+gt_status gt_context_link_stats(const gt_context* ctx, gt_link_stats* out) {
+    if (!ctx || !out) return fail(GT_E_INVALID, "NULL argument");
+    const LinkStats& l = ctx->last_link;
+    out->steps = l.steps;
+    out->pairs = l.pairs;
+    out->subnets = l.subnets;
+    out->max_subnet = l.max_subnet;
+    out->ids = l.ids;
+    out->search_ms = l.search_ms;
+    out->solve_ms = l.solve_ms;
     return GT_OK;
 }
 

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 namespace gt {
 
 // Output tile of the detect stage (K2).  It is also the grouping unit of
@@ -158,6 +161,22 @@ hipError_t launch_adjoint(const Batch& b, const Geom* d_geom, const AdjOut& ao,
 hipError_t launch_adjoint_vjp(const float* jac, const int32_t* origin, const uint64_t* frame_rows,
                               uint32_t nframes, const float* grad_rows, float* grad, int S, int H,
                               int W, hipStream_t s);
+
+// Linker (gt_link.hip).
+struct LinkStats {
+    uint64_t steps = 0, pairs = 0, subnets = 0, max_subnet = 0, ids = 0;
+    double search_ms = 0, solve_ms = 0;   // candidate search, subnet solver
+};
+// Particle ids of the n = step_off.back() positions (ys, xs: host arrays),
+// step s being positions [step_off[s], step_off[s + 1]).  On failure `err`
+// says why.
+hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
+                      const std::vector<uint32_t>& step_off, float search_range,
+                      uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err);
+// link_rust: steps are FrameSubsetter runs of column 0, y = column 2, x = 3.
+hipError_t link_rows(hipStream_t st, const float* rows, uint64_t n_rows, uint32_t ncols,
+                     float search_range, uint32_t memory, uint64_t* ids, LinkStats* stats,
+                     std::string& err);
 
 // Synthetic frames (gt_synth.hip).
 hipError_t launch_synth(uint16_t* frames, uint64_t nframes, uint32_t h,

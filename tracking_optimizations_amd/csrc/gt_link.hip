@@ -1,0 +1,445 @@
+// gt_link.hip -- frame-to-frame linker (gpu-tracking/src/linking.rs) with
+// the candidate search on the GPU.
+//
+// The caller hands over the positions of all steps at once (a step is one
+// call of Linker::advance: a FrameSubsetter run of rows for link_rust, one
+// frame for locate with search_range).  The particles of every step are
+// sorted by y on the device (rocPRIM segmented radix sort on
+// order-preserving keys); one thread per (particle p of step s, target step
+// s + 1 + j, j <= memory) binary-searches the target step's y band and
+// applies within_radius_rd's rule (linking.rs:73-105): the per-axis box
+// [q - r, q + r] with binary32 bounds, then (0 + dy*dy) + dx*dx < r*r in
+// binary32, dy = dest - src.  Two passes (count, exclusive scan, write) give
+// every (p, j) its exact candidate list, so a memory entry of any age finds
+// its candidates in the same table.
+//
+// The subnet solver then runs on the host in the order of Linker::advance /
+// link (linking.rs:234-492): memory entries appended oldest first, depth-
+// first subnets from each destination in order, one-to-one pairs taken
+// directly, exhaustive minimum-cost assignment (null cost r*r, first
+// strictly better score kept, binary32 running score), new ids in
+// destination order, unmatched sources of the previous step kept as memory.
+// A source's candidates are taken in ascending destination order; the
+// reference takes its kd-tree's traversal order, which only decides between
+// assignments of equal binary32 cost.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include "gt_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace gt {
+
+namespace {
+
+// Order-preserving key of a float (+0 and -0 share a key).
+__device__ __forceinline__ uint32_t ykey(float y) {
+    const uint32_t b = __float_as_uint(y + 0.0f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__global__ void k_link_keys(const float* __restrict__ ys, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        keys[i] = ykey(ys[i]);
+        vals[i] = i;
+    }
+}
+
+// One thread per (source particle p, age j).  Without `off` it counts the
+// candidates into counts[t]; with `off` it writes them from off[t] on, as
+// (destination index within the target step, squared distance).
+__global__ void k_link_pairs(const float* __restrict__ ys, const float* __restrict__ xs,
+                             const uint32_t* __restrict__ step_of,
+                             const uint32_t* __restrict__ step_off, uint32_t nsteps,
+                             const uint32_t* __restrict__ skeys,
+                             const uint32_t* __restrict__ sidx, uint32_t nm, float r,
+                             uint32_t n, uint32_t* __restrict__ counts,
+                             const uint32_t* __restrict__ off, uint32_t* __restrict__ out_d,
+                             float* __restrict__ out_dist) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)n * nm) return;
+    const uint32_t p = (uint32_t)(t / nm), j = (uint32_t)(t % nm);
+    const uint32_t ts = step_of[p] + 1 + j;
+    uint32_t c = 0;
+    if (ts < nsteps) {
+        const float sy = ys[p], sx = xs[p];
+        const float lo0 = sy - r, hi0 = sy + r, lo1 = sx - r, hi1 = sx + r;
+        const float r2 = r * r;
+        const uint32_t klo = ykey(lo0), khi = ykey(hi0);
+        const uint32_t a = step_off[ts], e = step_off[ts + 1];
+        uint32_t lo = a, hi = e;   // first key >= klo
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (skeys[mid] < klo) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t base = off ? off[t] : 0;
+        for (uint32_t k = lo; k < e && skeys[k] <= khi; ++k) {
+            const uint32_t d = sidx[k];
+            const float dy = ys[d], dx = xs[d];
+            if (dy < lo0 || dy > hi0 || dx < lo1 || dx > hi1) continue;
+            const float a0 = dy - sy, a1 = dx - sx;
+            float dist = 0.0f + a0 * a0;
+            dist = dist + a1 * a1;
+            if (dist < r2) {
+                if (off) {
+                    out_d[base + c] = d - a;
+                    out_dist[base + c] = dist;
+                }
+                ++c;
+            }
+        }
+    }
+    if (!off) counts[t] = c;
+}
+
+struct Cand {
+    uint32_t d;
+    float dist;
+};
+
+// A source of link(): the particle it was (index into the positions), the
+// step that particle belongs to, and its id.
+struct Src {
+    uint32_t p, s;
+    uint64_t id;
+};
+
+// recurse2 (linking.rs:391-452) over one subnet.
+struct Assign {
+    const std::vector<uint32_t>* dests = nullptr;
+    const std::vector<std::vector<std::pair<uint32_t, float>>>* d2s = nullptr;
+    const std::vector<Src>* srcs = nullptr;
+    std::vector<int64_t>* out = nullptr;   // -1: no source
+    std::vector<uint64_t> used_id;
+    std::vector<uint32_t> used_dest;
+    std::vector<uint32_t> nulls;
+    float score = 0.f, best = 0.f, null_cost = 0.f;
+
+    bool in_use(uint64_t id) const {
+        return std::find(used_id.begin(), used_id.end(), id) != used_id.end();
+    }
+
+    void run(size_t progress) {
+        const uint32_t cur = (*dests)[progress];
+        const auto& cands = (*d2s)[cur];
+        for (size_t q = 0; q <= cands.size(); ++q) {
+            const bool is_null = q == cands.size();
+            const float dist = is_null ? null_cost : cands[q].second;
+            if (score > best) return;
+            score = score + dist;
+            if (!is_null) {
+                const uint64_t sid = (*srcs)[cands[q].first].id;
+                if (in_use(sid)) {
+                    score = score - dist;
+                    continue;
+                }
+                used_id.push_back(sid);
+                used_dest.push_back(cur);
+            } else {
+                nulls.push_back(cur);
+            }
+            if (progress + 1 == dests->size()) {
+                if (score < best) {
+                    best = score;
+                    for (size_t u = 0; u < used_id.size(); ++u)
+                        (*out)[used_dest[u]] = (int64_t)used_id[u];
+                    for (uint32_t v : nulls) (*out)[v] = -1;
+                }
+            } else {
+                run(progress + 1);
+            }
+            if (!is_null) {
+                used_id.pop_back();
+                used_dest.pop_back();
+            } else {
+                nulls.pop_back();
+            }
+            score = score - dist;
+        }
+    }
+};
+
+struct DevArena {
+    std::vector<void*> ptrs;
+    ~DevArena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T> hipError_t alloc(T** p, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
+        if (e == hipSuccess) ptrs.push_back(q);
+        *p = (T*)q;
+        return e;
+    }
+};
+
+}  // namespace
+
+#define LK_TRY(x)                                                 \
+    do {                                                          \
+        const hipError_t e_ = (x);                                \
+        if (e_ != hipSuccess) {                                   \
+            err = std::string(#x) + ": " + hipGetErrorString(e_); \
+            return e_;                                            \
+        }                                                         \
+    } while (0)
+
+hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
+                      const std::vector<uint32_t>& step_off, float search_range,
+                      uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err) {
+    const uint32_t nsteps = (uint32_t)step_off.size() - 1;
+    const uint32_t n = step_off.back();
+    if (stats) *stats = LinkStats{};
+    // memory beyond the number of steps behaves as memory == steps: entries
+    // never expire within the call
+    memory = std::min(memory, nsteps);
+    const uint32_t nm = memory + 1;
+    const size_t nt = (size_t)n * nm;
+    if (nt >= (1ull << 32)) {
+        err = "link: rows * (memory + 1) must be below 2^32";
+        return hipErrorInvalidValue;
+    }
+
+    if (stats) stats->steps = nsteps;
+    const auto t0 = std::chrono::steady_clock::now();
+
+    // ---- candidates on the device --------------------------------------------
+    std::vector<uint32_t> off_h(nt + 1, 0);
+    std::vector<uint32_t> cd_h;
+    std::vector<float> cdist_h;
+    if (n > 0 && nsteps > 1) {
+        std::vector<uint32_t> step_of(n);
+        for (uint32_t s = 0; s < nsteps; ++s)
+            for (uint32_t i = step_off[s]; i < step_off[s + 1]; ++i) step_of[i] = s;
+        DevArena B;
+        float *d_y, *d_x, *d_dist;
+        uint32_t *d_stepof, *d_stepoff, *d_k0, *d_v0, *d_k1, *d_v1, *d_cnt, *d_off, *d_d;
+        char *d_sort_tmp, *d_scan_tmp;
+        LK_TRY(B.alloc(&d_y, n));
+        LK_TRY(B.alloc(&d_x, n));
+        LK_TRY(B.alloc(&d_stepof, n));
+        LK_TRY(B.alloc(&d_stepoff, nsteps + 1));
+        LK_TRY(B.alloc(&d_k0, n));
+        LK_TRY(B.alloc(&d_v0, n));
+        LK_TRY(B.alloc(&d_k1, n));
+        LK_TRY(B.alloc(&d_v1, n));
+        LK_TRY(B.alloc(&d_cnt, nt));
+        LK_TRY(B.alloc(&d_off, nt));
+        LK_TRY(hipMemcpyAsync(d_y, ys, (size_t)n * 4, hipMemcpyHostToDevice, st));
+        LK_TRY(hipMemcpyAsync(d_x, xs, (size_t)n * 4, hipMemcpyHostToDevice, st));
+        LK_TRY(hipMemcpyAsync(d_stepof, step_of.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+        LK_TRY(hipMemcpyAsync(d_stepoff, step_off.data(), (size_t)(nsteps + 1) * 4,
+                              hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_link_keys, dim3((n + 255) / 256), dim3(256), 0, st, d_y, d_k0, d_v0,
+                           n);
+        LK_TRY(hipGetLastError());
+        size_t sort_bytes = 0, scan_bytes = 0;
+        LK_TRY(rocprim::segmented_radix_sort_pairs(nullptr, sort_bytes, d_k0, d_k1, d_v0, d_v1, n,
+                                                   nsteps, d_stepoff, d_stepoff + 1, 0, 32, st));
+        LK_TRY(B.alloc(&d_sort_tmp, sort_bytes));
+        LK_TRY(rocprim::segmented_radix_sort_pairs((void*)d_sort_tmp, sort_bytes, d_k0, d_k1, d_v0,
+                                                   d_v1, n, nsteps, d_stepoff, d_stepoff + 1, 0,
+                                                   32, st));
+        const uint32_t blocks = (uint32_t)((nt + 255) / 256);
+        hipLaunchKernelGGL(k_link_pairs, dim3(blocks), dim3(256), 0, st, d_y, d_x, d_stepof,
+                           d_stepoff, nsteps, d_k1, d_v1, nm, search_range, n, d_cnt,
+                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (float*)nullptr);
+        LK_TRY(hipGetLastError());
+        LK_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, d_cnt, d_off, 0u, nt,
+                                       rocprim::plus<uint32_t>(), st));
+        LK_TRY(B.alloc(&d_scan_tmp, scan_bytes));
+        LK_TRY(rocprim::exclusive_scan((void*)d_scan_tmp, scan_bytes, d_cnt, d_off, 0u, nt,
+                                       rocprim::plus<uint32_t>(), st));
+        uint32_t last = 0;
+        LK_TRY(hipMemcpyAsync(off_h.data(), d_off, nt * 4, hipMemcpyDeviceToHost, st));
+        LK_TRY(hipMemcpyAsync(&last, d_cnt + nt - 1, 4, hipMemcpyDeviceToHost, st));
+        LK_TRY(hipStreamSynchronize(st));
+        const uint64_t total = (uint64_t)off_h[nt - 1] + last;
+        if (total >= (1ull << 32)) {
+            err = "link: more than 2^32 - 1 candidate pairs";
+            return hipErrorInvalidValue;
+        }
+        off_h[nt] = (uint32_t)total;
+        cd_h.resize(total);
+        cdist_h.resize(total);
+        if (total) {
+            LK_TRY(B.alloc(&d_d, total));
+            LK_TRY(B.alloc(&d_dist, total));
+            hipLaunchKernelGGL(k_link_pairs, dim3(blocks), dim3(256), 0, st, d_y, d_x, d_stepof,
+                               d_stepoff, nsteps, d_k1, d_v1, nm, search_range, n, d_cnt,
+                               (const uint32_t*)d_off, d_d, d_dist);
+            LK_TRY(hipGetLastError());
+            LK_TRY(hipMemcpyAsync(cd_h.data(), d_d, total * 4, hipMemcpyDeviceToHost, st));
+            LK_TRY(hipMemcpyAsync(cdist_h.data(), d_dist, total * 4, hipMemcpyDeviceToHost, st));
+            LK_TRY(hipStreamSynchronize(st));
+        }
+        if (stats) stats->pairs = total;
+    }
+
+    const auto t1 = std::chrono::steady_clock::now();
+
+    // ---- Linker::advance over the steps --------------------------------------
+    std::vector<Src> prev;
+    uint32_t prev_n = 0;
+    std::vector<std::vector<Src>> memvec(memory);   // front = oldest
+    uint64_t counter = 0;
+    const float r = search_range;
+    std::vector<std::vector<Cand>> s2d;
+    std::vector<std::vector<std::pair<uint32_t, float>>> d2s;
+    std::vector<uint8_t> vis_s, vis_d;
+    std::vector<std::pair<uint32_t, uint8_t>> stack;
+    std::vector<uint32_t> path_s, path_d;
+    std::vector<std::vector<uint32_t>> subnets;
+    std::vector<int64_t> out;
+    std::vector<uint64_t> present, added, sorted_ids;
+    Assign A;
+    A.null_cost = r * r;
+    uint64_t n_subnets = 0, max_subnet = 0;
+    for (uint32_t t = 0; t < nsteps; ++t) {
+        const uint32_t a = step_off[t], nd = step_off[t + 1] - step_off[t];
+        // memory entries, oldest first, ids not already among the sources
+        if (memory > 0) {
+            present.clear();
+            for (const Src& s : prev) present.push_back(s.id);
+            std::sort(present.begin(), present.end());
+            added.clear();
+            for (const auto& m : memvec)
+                for (const Src& e : m) {
+                    if (std::binary_search(present.begin(), present.end(), e.id) ||
+                        std::find(added.begin(), added.end(), e.id) != added.end())
+                        continue;
+                    prev.push_back(e);
+                    added.push_back(e.id);
+                }
+        }
+        // candidate lists in ascending destination order
+        const size_t ns = prev.size();
+        s2d.assign(ns, {});
+        d2s.assign(nd, {});
+        for (size_t si = 0; si < ns; ++si) {
+            const Src& s = prev[si];
+            const uint32_t j = t - s.s - 1;
+            if (j >= nm) continue;
+            const size_t key = (size_t)s.p * nm + j;
+            auto& lst = s2d[si];
+            for (uint32_t q = off_h[key]; q < off_h[key + 1]; ++q) lst.push_back({cd_h[q], cdist_h[q]});
+            std::sort(lst.begin(), lst.end(), [](const Cand& x, const Cand& y) { return x.d < y.d; });
+            for (const Cand& c : lst) d2s[c.d].push_back({(uint32_t)si, c.dist});
+        }
+        // connected subnets, depth first from each destination in order
+        // (the recursion of linking.rs:317-341 with an explicit stack)
+        vis_s.assign(ns, 0);
+        vis_d.assign(nd, 0);
+        out.assign(nd, -1);
+        subnets.clear();
+        for (uint32_t i = 0; i < nd; ++i) {
+            path_s.clear();
+            path_d.clear();
+            stack.assign(1, {i, 1});
+            while (!stack.empty()) {
+                const auto [idx, kind] = stack.back();
+                stack.pop_back();
+                if (kind == 0) {
+                    if (vis_s[idx]) continue;
+                    vis_s[idx] = 1;
+                    path_s.push_back(idx);
+                    const auto& lst = s2d[idx];
+                    for (size_t q = lst.size(); q-- > 0;) stack.push_back({lst[q].d, 1});
+                } else {
+                    if (vis_d[idx]) continue;
+                    vis_d[idx] = 1;
+                    path_d.push_back(idx);
+                    const auto& lst = d2s[idx];
+                    for (size_t q = lst.size(); q-- > 0;) stack.push_back({lst[q].first, 0});
+                }
+            }
+            if (path_d.size() == 1 && path_s.size() == 1) out[i] = (int64_t)prev[path_s[0]].id;
+            else if (path_d.size() > 1 || (path_d.size() == 1 && path_s.size() > 1))
+                subnets.push_back(path_d);
+        }
+        A.d2s = &d2s;
+        A.srcs = &prev;
+        A.out = &out;
+        for (const auto& dv : subnets) {
+            A.dests = &dv;
+            A.used_id.clear();
+            A.used_dest.clear();
+            A.nulls.clear();
+            A.score = 0.f;
+            A.best = __builtin_inff();
+            A.run(0);
+            ++n_subnets;
+            max_subnet = std::max<uint64_t>(max_subnet, dv.size());
+        }
+        // ids; destinations without a source get new ones in order
+        std::vector<Src> next(nd);
+        sorted_ids.resize(nd);
+        for (uint32_t i = 0; i < nd; ++i) {
+            const uint64_t id = out[i] >= 0 ? (uint64_t)out[i] : counter++;
+            ids[a + i] = id;
+            sorted_ids[i] = id;
+            next[i] = {a + i, t, id};
+        }
+        if (memory > 0) {
+            std::sort(sorted_ids.begin(), sorted_ids.end());
+            std::vector<Src> unused;
+            for (uint32_t si = 0; si < prev_n; ++si)
+                if (!std::binary_search(sorted_ids.begin(), sorted_ids.end(), prev[si].id))
+                    unused.push_back(prev[si]);
+            memvec.erase(memvec.begin());
+            memvec.push_back(std::move(unused));
+        }
+        prev = std::move(next);
+        prev_n = nd;
+    }
+    if (stats) {
+        using ms = std::chrono::duration<double, std::milli>;
+        stats->search_ms = ms(t1 - t0).count();
+        stats->solve_ms = ms(std::chrono::steady_clock::now() - t1).count();
+        stats->subnets = n_subnets;
+        stats->max_subnet = max_subnet;
+        stats->ids = counter;
+    }
+    return hipSuccess;
+}
+
+hipError_t link_rows(hipStream_t st, const float* rows, uint64_t n_rows, uint32_t ncols,
+                     float search_range, uint32_t memory, uint64_t* ids, LinkStats* stats,
+                     std::string& err) {
+    if (n_rows >= (1ull << 32)) {
+        err = "link: at most 2^32 - 1 rows";
+        return hipErrorInvalidValue;
+    }
+    // FrameSubsetter (linking.rs:34-58): a new step starts where the frame
+    // value differs from the previous one, the first comparison being
+    // against 0.0 (a first frame other than 0 yields an empty first step);
+    // no rows, no steps.
+    std::vector<uint32_t> step_off(1, 0);
+    std::vector<float> ys(n_rows), xs(n_rows);
+    float cur = 0.0f;
+    for (uint64_t i = 0; i < n_rows; ++i) {
+        const float fr = rows[i * ncols + 0];
+        if (fr != cur) {
+            cur = fr;
+            step_off.push_back((uint32_t)i);
+        }
+        ys[i] = rows[i * ncols + 2];
+        xs[i] = rows[i * ncols + 3];
+    }
+    if (n_rows > 0) step_off.push_back((uint32_t)n_rows);
+    return link_steps(st, ys.data(), xs.data(), step_off, search_range, memory, ids, stats, err);
+}
+
+#undef LK_TRY
+
+}  // namespace gt

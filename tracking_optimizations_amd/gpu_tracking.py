@@ -15,10 +15,12 @@ Differences, all supersets:
     numpy array or a torch tensor (device tensors are read in place);
   * errors raise ``GtInvalid`` (a ValueError) or ``GtNotImplemented`` (a
     NotImplementedError) instead of a Rust panic.
-Not implemented in this build (raise GtNotImplemented): ``search_range``
-(linking), ``sig_radius``/``bg_radius`` (raw signal extraction) and the
-parameters the reference itself rejects (maxsize, threshold, invert, topn,
-preprocess).
+``search_range``/``memory`` link the particles frame to frame as
+execute_gpu.rs:335 does (a ``particle`` column after ``x``), and
+``link_rust(array, search_range, memory)`` links a row table as lib.rs:202-212
+does.  Not implemented in this build (raise GtNotImplemented):
+``sig_radius``/``bg_radius`` (raw signal extraction) and the parameters the
+reference itself rejects (maxsize, threshold, invert, topn, preprocess).
 """
 from __future__ import annotations
 
@@ -32,7 +34,7 @@ from ._lib import (GT_FRAMES_ON_DEVICE, GT_PLAIN, GtError, GtInvalid, GtNotImple
                    GtParams, Result, check, default_context, load)
 
 __all__ = ["batch", "batch_rust", "batch_file", "batch_file_rust", "make_params",
-           "locate", "column_names", "GtError", "GtInvalid", "GtNotImplemented"]
+           "locate", "column_names", "link_rust", "GtError", "GtInvalid", "GtNotImplemented"]
 
 
 def make_params(diameter, minmass=None, maxsize=None, separation=None, noise_size=None,
@@ -150,6 +152,22 @@ def locate(video, params: GtParams, plain: bool = False, device: int = 0, ctx=No
     del keep
     res = Result(out)
     return res.rows(), res.candidates(), res.columns(), ctx.timing()
+
+
+def link_rust(array, search_range, memory=None, device: int = 0, ctx=None) -> np.ndarray:
+    """lib.rs:202-212: particle id (uint64) of every row of a float32 table
+    whose column 0 is the frame and columns 2, 3 are y, x; steps are runs of
+    equal frame values (FrameSubsetter, linking.rs:12-61)."""
+    arr = np.ascontiguousarray(np.asarray(array, dtype=np.float32))
+    if arr.ndim != 2:
+        raise GtInvalid(_lib.GT_E_INVALID, "link_rust takes a 2-d array")
+    n, c = arr.shape
+    ids = np.empty(n, np.uint64)
+    ctx = ctx or default_context(device)
+    check(load().gt_link(ctx.handle, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n, c,
+                         float(search_range), int(memory or 0),
+                         ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))))
+    return ids
 
 
 def batch_rust(video, diameter, minmass=None, maxsize=None, separation=None, noise_size=None,

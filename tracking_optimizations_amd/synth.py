@@ -13,7 +13,11 @@ Workloads (sizes and parameters of the benchmark configurations):
     per axis), reflecting walls;
   * ``uneven``:      mixed 256x256 / 512x1024 / 2048x2048 frames (40/40/20 %),
     one spot per 800 px^2, sigma U(1.2, 2.5), bg U(50, 300) per frame;
-  * ``adjoint_512``: 512x512, bg 100, 300 spots, sigma 1.5, amp U(200, 3000).
+  * ``adjoint_512``: 512x512, bg 100, 300 spots, sigma 1.5, amp U(200, 3000);
+  * ``link_1024``:   1024x1024, bg 120, 3000 spots, sigma 1.8, amp lognormal
+    (median 1200, s 0.5), Brownian D = 0.5, each spot off in a frame with
+    probability 0.05 independently (``MovieSpec.track`` holds the spot index
+    of every drawn spot).
 """
 from __future__ import annotations
 
@@ -34,6 +38,7 @@ class MovieSpec:
     spot_off: np.ndarray   # (n_frames + 1,) uint64
     bg: np.ndarray         # (n_frames,) float32
     seed: int
+    track: np.ndarray | None = None   # (n_spots,) spot index of each drawn spot
 
     @property
     def n_frames(self) -> int:
@@ -92,6 +97,29 @@ def movie_2048(seed: int = 2, n_frames: int = 1000, n_spots: int = 5000, size: i
     spots = np.concatenate(frames).astype(np.float32)
     off = np.arange(n_frames + 1, dtype=np.uint64) * n_spots
     return MovieSpec(size, size, spots, off, np.full(n_frames, bg, np.float32), seed)
+
+
+def link_1024(seed: int = 5, n_frames: int = 2000, n_spots: int = 3000, size: int = 1024,
+              sigma: float = 1.8, bg: float = 120.0, D: float = 0.5,
+              p_off: float = 0.05) -> MovieSpec:
+    """Brownian spots that blink off independently per frame."""
+    rng = np.random.default_rng(seed)
+    pos = np.column_stack([rng.uniform(0, size - 1, n_spots), rng.uniform(0, size - 1, n_spots)])
+    amp = np.exp(rng.normal(np.log(1200.0), 0.5, n_spots))
+    step = np.sqrt(2 * D)
+    frames, tracks, off = [], [], [0]
+    idx = np.arange(n_spots)
+    for _ in range(n_frames):
+        on = rng.random(n_spots) >= p_off
+        frames.append(np.column_stack([pos[on], amp[on], np.full(int(on.sum()), sigma)]))
+        tracks.append(idx[on])
+        off.append(off[-1] + int(on.sum()))
+        pos = pos + rng.normal(0, step, pos.shape)
+        pos = np.where(pos < 0, -pos, pos)
+        pos = np.where(pos > size - 1, 2 * (size - 1) - pos, pos)
+    return MovieSpec(size, size, np.concatenate(frames).astype(np.float32),
+                     np.array(off, np.uint64), np.full(n_frames, bg, np.float32), seed,
+                     np.concatenate(tracks))
 
 
 def uneven_shapes(seed: int = 3, n_frames: int = 4096):
