@@ -102,11 +102,6 @@ __global__ void k_link_pairs(const float* __restrict__ ys, const float* __restri
     if (!off) counts[t] = c;
 }
 
-struct Cand {
-    uint32_t d;
-    float dist;
-};
-
 // A source of link(): the particle it was (index into the positions), the
 // step that particle belongs to, and its id.
 struct Src {
@@ -114,12 +109,18 @@ struct Src {
     uint64_t id;
 };
 
-// recurse2 (linking.rs:391-452) over one subnet.
+// recurse2 (linking.rs:391-452) over one subnet: every destination in turn
+// takes one of its sources (ascending source order) or none (cost r*r); the
+// running binary32 score is compared with the best one before each choice
+// and the first assignment with a strictly smaller total is kept.
 struct Assign {
-    const std::vector<uint32_t>* dests = nullptr;
-    const std::vector<std::vector<std::pair<uint32_t, float>>>* d2s = nullptr;
-    const std::vector<Src>* srcs = nullptr;
-    std::vector<int64_t>* out = nullptr;   // -1: no source
+    const uint32_t* dests = nullptr;   // destinations of the subnet, DFS order
+    size_t ndests = 0;
+    const uint32_t* d2s_off = nullptr; // CSR of destination -> (source, distance)
+    const uint32_t* d2s_src = nullptr;
+    const float* d2s_dist = nullptr;
+    const Src* srcs = nullptr;
+    int64_t* out = nullptr;            // -1: no source
     std::vector<uint64_t> used_id;
     std::vector<uint32_t> used_dest;
     std::vector<uint32_t> nulls;
@@ -130,15 +131,15 @@ struct Assign {
     }
 
     void run(size_t progress) {
-        const uint32_t cur = (*dests)[progress];
-        const auto& cands = (*d2s)[cur];
-        for (size_t q = 0; q <= cands.size(); ++q) {
-            const bool is_null = q == cands.size();
-            const float dist = is_null ? null_cost : cands[q].second;
+        const uint32_t cur = dests[progress];
+        const uint32_t b = d2s_off[cur], e = d2s_off[cur + 1];
+        for (uint32_t q = b; q <= e; ++q) {
+            const bool is_null = q == e;
+            const float dist = is_null ? null_cost : d2s_dist[q];
             if (score > best) return;
             score = score + dist;
             if (!is_null) {
-                const uint64_t sid = (*srcs)[cands[q].first].id;
+                const uint64_t sid = srcs[d2s_src[q]].id;
                 if (in_use(sid)) {
                     score = score - dist;
                     continue;
@@ -148,12 +149,11 @@ struct Assign {
             } else {
                 nulls.push_back(cur);
             }
-            if (progress + 1 == dests->size()) {
+            if (progress + 1 == ndests) {
                 if (score < best) {
                     best = score;
-                    for (size_t u = 0; u < used_id.size(); ++u)
-                        (*out)[used_dest[u]] = (int64_t)used_id[u];
-                    for (uint32_t v : nulls) (*out)[v] = -1;
+                    for (size_t u = 0; u < used_id.size(); ++u) out[used_dest[u]] = (int64_t)used_id[u];
+                    for (uint32_t v : nulls) out[v] = -1;
                 }
             } else {
                 run(progress + 1);
@@ -289,62 +289,91 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
     const auto t1 = std::chrono::steady_clock::now();
 
     // ---- Linker::advance over the steps --------------------------------------
-    std::vector<Src> prev;
+    // Ids are dense (< n); a per-id stamp answers "is this id among the
+    // sources" (2t + 1) and "was it handed to a destination" (2t + 2).
+    std::vector<uint32_t> stamp((size_t)n + 1, 0);
+    std::vector<Src> prev, next;
     uint32_t prev_n = 0;
-    std::vector<std::vector<Src>> memvec(memory);   // front = oldest
+    std::vector<std::vector<Src>> ring(memory);   // memory steps, ring[head] oldest
+    uint32_t head = 0;
     uint64_t counter = 0;
-    const float r = search_range;
-    std::vector<std::vector<Cand>> s2d;
-    std::vector<std::vector<std::pair<uint32_t, float>>> d2s;
+    std::vector<uint32_t> d2s_off, d2s_fill, d2s_src;
+    std::vector<float> d2s_dist;
     std::vector<uint8_t> vis_s, vis_d;
     std::vector<std::pair<uint32_t, uint8_t>> stack;
-    std::vector<uint32_t> path_s, path_d;
-    std::vector<std::vector<uint32_t>> subnets;
+    std::vector<uint32_t> path_d, sub_dests, sub_off;
     std::vector<int64_t> out;
-    std::vector<uint64_t> present, added, sorted_ids;
     Assign A;
-    A.null_cost = r * r;
+    A.null_cost = search_range * search_range;
     uint64_t n_subnets = 0, max_subnet = 0;
+    auto list_of = [&](const Src& s, uint32_t t, uint32_t& b, uint32_t& e) {
+        const uint32_t j = t - s.s - 1;
+        if (j >= nm) {
+            b = e = 0;
+            return;
+        }
+        const size_t key = (size_t)s.p * nm + j;
+        b = off_h[key];
+        e = off_h[key + 1];
+    };
     for (uint32_t t = 0; t < nsteps; ++t) {
         const uint32_t a = step_off[t], nd = step_off[t + 1] - step_off[t];
+        const uint32_t st_src = 2 * t + 1, st_out = 2 * t + 2;
         // memory entries, oldest first, ids not already among the sources
         if (memory > 0) {
-            present.clear();
-            for (const Src& s : prev) present.push_back(s.id);
-            std::sort(present.begin(), present.end());
-            added.clear();
-            for (const auto& m : memvec)
-                for (const Src& e : m) {
-                    if (std::binary_search(present.begin(), present.end(), e.id) ||
-                        std::find(added.begin(), added.end(), e.id) != added.end())
-                        continue;
+            for (const Src& s : prev) stamp[s.id] = st_src;
+            for (uint32_t k = 0; k < memory; ++k)
+                for (const Src& e : ring[(head + k) % memory]) {
+                    if (stamp[e.id] == st_src) continue;
+                    stamp[e.id] = st_src;
                     prev.push_back(e);
-                    added.push_back(e.id);
                 }
         }
-        // candidate lists in ascending destination order
-        const size_t ns = prev.size();
-        s2d.assign(ns, {});
-        d2s.assign(nd, {});
-        for (size_t si = 0; si < ns; ++si) {
-            const Src& s = prev[si];
-            const uint32_t j = t - s.s - 1;
-            if (j >= nm) continue;
-            const size_t key = (size_t)s.p * nm + j;
-            auto& lst = s2d[si];
-            for (uint32_t q = off_h[key]; q < off_h[key + 1]; ++q) lst.push_back({cd_h[q], cdist_h[q]});
-            std::sort(lst.begin(), lst.end(), [](const Cand& x, const Cand& y) { return x.d < y.d; });
-            for (const Cand& c : lst) d2s[c.d].push_back({(uint32_t)si, c.dist});
+        const uint32_t ns = (uint32_t)prev.size();
+        // every source's list in ascending destination order (each list
+        // belongs to one step only, so it is sorted in place), and the
+        // destination -> source lists in ascending source order
+        d2s_off.assign((size_t)nd + 1, 0);
+        for (uint32_t si = 0; si < ns; ++si) {
+            uint32_t b, e;
+            list_of(prev[si], t, b, e);
+            for (uint32_t q = b + 1; q < e; ++q) {   // insertion sort, lists are short
+                const uint32_t d = cd_h[q];
+                const float v = cdist_h[q];
+                uint32_t k = q;
+                for (; k > b && cd_h[k - 1] > d; --k) {
+                    cd_h[k] = cd_h[k - 1];
+                    cdist_h[k] = cdist_h[k - 1];
+                }
+                cd_h[k] = d;
+                cdist_h[k] = v;
+            }
+            for (uint32_t q = b; q < e; ++q) ++d2s_off[cd_h[q] + 1];
+        }
+        for (uint32_t d = 0; d < nd; ++d) d2s_off[d + 1] += d2s_off[d];
+        d2s_fill.assign(d2s_off.begin(), d2s_off.end() - 1);
+        d2s_src.resize(d2s_off[nd]);
+        d2s_dist.resize(d2s_off[nd]);
+        for (uint32_t si = 0; si < ns; ++si) {
+            uint32_t b, e;
+            list_of(prev[si], t, b, e);
+            for (uint32_t q = b; q < e; ++q) {
+                const uint32_t w = d2s_fill[cd_h[q]]++;
+                d2s_src[w] = si;
+                d2s_dist[w] = cdist_h[q];
+            }
         }
         // connected subnets, depth first from each destination in order
         // (the recursion of linking.rs:317-341 with an explicit stack)
         vis_s.assign(ns, 0);
         vis_d.assign(nd, 0);
         out.assign(nd, -1);
-        subnets.clear();
+        sub_dests.clear();
+        sub_off.assign(1, 0);
         for (uint32_t i = 0; i < nd; ++i) {
-            path_s.clear();
+            if (vis_d[i]) continue;
             path_d.clear();
+            uint32_t n_src = 0, first_src = 0;
             stack.assign(1, {i, 1});
             while (!stack.empty()) {
                 const auto [idx, kind] = stack.back();
@@ -352,26 +381,33 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
                 if (kind == 0) {
                     if (vis_s[idx]) continue;
                     vis_s[idx] = 1;
-                    path_s.push_back(idx);
-                    const auto& lst = s2d[idx];
-                    for (size_t q = lst.size(); q-- > 0;) stack.push_back({lst[q].d, 1});
+                    if (n_src++ == 0) first_src = idx;
+                    uint32_t b, e;
+                    list_of(prev[idx], t, b, e);
+                    for (uint32_t q = e; q-- > b;) stack.push_back({cd_h[q], 1});
                 } else {
                     if (vis_d[idx]) continue;
                     vis_d[idx] = 1;
                     path_d.push_back(idx);
-                    const auto& lst = d2s[idx];
-                    for (size_t q = lst.size(); q-- > 0;) stack.push_back({lst[q].first, 0});
+                    for (uint32_t q = d2s_off[idx + 1]; q-- > d2s_off[idx];)
+                        stack.push_back({d2s_src[q], 0});
                 }
             }
-            if (path_d.size() == 1 && path_s.size() == 1) out[i] = (int64_t)prev[path_s[0]].id;
-            else if (path_d.size() > 1 || (path_d.size() == 1 && path_s.size() > 1))
-                subnets.push_back(path_d);
+            if (path_d.size() == 1 && n_src == 1) {
+                out[i] = (int64_t)prev[first_src].id;
+            } else if (path_d.size() > 1 || n_src > 1) {
+                sub_dests.insert(sub_dests.end(), path_d.begin(), path_d.end());
+                sub_off.push_back((uint32_t)sub_dests.size());
+            }
         }
-        A.d2s = &d2s;
-        A.srcs = &prev;
-        A.out = &out;
-        for (const auto& dv : subnets) {
-            A.dests = &dv;
+        A.d2s_off = d2s_off.data();
+        A.d2s_src = d2s_src.data();
+        A.d2s_dist = d2s_dist.data();
+        A.srcs = prev.data();
+        A.out = out.data();
+        for (size_t k = 0; k + 1 < sub_off.size(); ++k) {
+            A.dests = sub_dests.data() + sub_off[k];
+            A.ndests = sub_off[k + 1] - sub_off[k];
             A.used_id.clear();
             A.used_dest.clear();
             A.nulls.clear();
@@ -379,27 +415,26 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
             A.best = __builtin_inff();
             A.run(0);
             ++n_subnets;
-            max_subnet = std::max<uint64_t>(max_subnet, dv.size());
+            max_subnet = std::max<uint64_t>(max_subnet, A.ndests);
         }
         // ids; destinations without a source get new ones in order
-        std::vector<Src> next(nd);
-        sorted_ids.resize(nd);
+        next.resize(nd);
         for (uint32_t i = 0; i < nd; ++i) {
             const uint64_t id = out[i] >= 0 ? (uint64_t)out[i] : counter++;
             ids[a + i] = id;
-            sorted_ids[i] = id;
+            stamp[id] = st_out;
             next[i] = {a + i, t, id};
         }
         if (memory > 0) {
-            std::sort(sorted_ids.begin(), sorted_ids.end());
-            std::vector<Src> unused;
+            // unmatched particles of the previous step replace the oldest
+            // memory step
+            std::vector<Src>& slot = ring[head];
+            slot.clear();
             for (uint32_t si = 0; si < prev_n; ++si)
-                if (!std::binary_search(sorted_ids.begin(), sorted_ids.end(), prev[si].id))
-                    unused.push_back(prev[si]);
-            memvec.erase(memvec.begin());
-            memvec.push_back(std::move(unused));
+                if (stamp[prev[si].id] != st_out) slot.push_back(prev[si]);
+            head = (head + 1) % memory;
         }
-        prev = std::move(next);
+        std::swap(prev, next);
         prev_n = nd;
     }
     if (stats) {
