@@ -93,7 +93,7 @@ def test_rust_defaults(lib):
 @pytest.mark.parametrize("kw,status", [
     (dict(maxsize=3.0), 2), (dict(threshold=0.1), 2), (dict(invert=True), 2),
     (dict(topn=5), 2), (dict(preprocess=False), 2), (dict(search_range=5.0), 0),
-    (dict(sig_radius=3.0), 2), (dict(bg_radius=8.0), 2),
+    (dict(sig_radius=3.0), 0), (dict(bg_radius=8.0), 1), (dict(sig_radius=3.0, bg_radius=8.0), 0),
     (dict(noise_size=0.0), 1), (dict(percentile=101.0), 1), (dict(percentile=-1.0), 1),
     (dict(smoothing_size=0), 1), (dict(percentile=64.0), 0), (dict(), 0),
 ])
@@ -158,7 +158,9 @@ def test_python_api_errors_before_device():
     with pytest.raises(GtNotImplemented):
         batch(v, 9, maxsize=3)
     with pytest.raises(GtNotImplemented):
-        batch(v, 9, sig_radius=4.0)
+        batch(v, 9, invert=True)
+    with pytest.raises(GtInvalid):
+        batch(v, 9, bg_radius=4.0)
     with pytest.raises(GtInvalid):
         batch(np.zeros((2, 32, 64), np.float32)[:, :, ::2], 9)
     with pytest.raises(GtInvalid):

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgpu_tracking.so")
-SOURCES = ["gt_kernels.hip", "gt_synth.hip", "gt_adjoint.hip", "gt_link.hip", "gt_api.hip"]
+SOURCES = ["gt_kernels.hip", "gt_synth.hip", "gt_adjoint.hip", "gt_link.hip", "gt_raw.hip", "gt_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

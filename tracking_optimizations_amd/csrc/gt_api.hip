@@ -133,9 +133,15 @@ gt_status check_params(const gt_params* p, uint32_t h, uint32_t w) {
     if (p->has_invert) return fail(GT_E_NOT_IMPLEMENTED, "invert is not implemented");
     if (p->has_topn) return fail(GT_E_NOT_IMPLEMENTED, "topn is not implemented");
     if (p->has_preprocess) return fail(GT_E_NOT_IMPLEMENTED, "preprocess is not implemented");
-    if (p->has_sig_radius || p->has_bg_radius)
-        return fail(GT_E_NOT_IMPLEMENTED,
-                    "sig_radius/bg_radius (raw signal extraction) is not implemented in this build");
+    // execute_gpu.rs:404-406,440: the background needs the signal radius (the
+    // reference declares the background columns but fills them only then)
+    if (p->has_bg_radius && !p->has_sig_radius)
+        return fail(GT_E_INVALID, "bg_radius needs sig_radius");
+    if (p->has_sig_radius && !std::isfinite(p->sig_radius))
+        return fail(GT_E_INVALID, "sig_radius must be finite");
+    if (p->has_bg_radius && (!std::isfinite(p->bg_radius) ||
+                             (p->has_gap_radius && !std::isfinite(p->gap_radius))))
+        return fail(GT_E_INVALID, "bg_radius and gap_radius must be finite");
     if (p->diameter < 1) return fail(GT_E_INVALID, "diameter must be >= 1");
     if (p->smoothing_size < 1) return fail(GT_E_INVALID, "smoothing_size must be >= 1");
     if (p->smoothing_size > 33) return fail(GT_E_INVALID, "smoothing_size must be <= 33");
@@ -162,8 +168,33 @@ gt_status check_params(const gt_params* p, uint32_t h, uint32_t w) {
     return GT_OK;
 }
 
+int32_t raw_cols_of(const gt_params* p) {
+    return p->has_sig_radius ? (p->has_bg_radius ? 3 : 1) : 0;
+}
+
 int32_t ncols_of(const gt_params* p) {
-    return 4 + (p->cpu_processed ? 3 : 0) + (p->characterize ? 4 : 0);
+    return 4 + (p->cpu_processed ? 3 : 0) + raw_cols_of(p) + (p->characterize ? 4 : 0);
+}
+
+// circle_inds (kernels.rs:149-164) and annulus_inds (kernels.rs:166-181):
+// offsets (i, j), i = row, in row-major order over the bounding square,
+// with the squared integer distance compared in binary32.
+void circle_taps(float radius, std::vector<std::pair<int, int>>& out) {
+    const int rb = (int)std::ceil(radius);
+    const float r2 = radius * radius;
+    for (int i = -rb; i <= rb; ++i)
+        for (int j = -rb; j <= rb; ++j)
+            if ((float)(i * i + j * j) <= r2) out.push_back({i, j});
+}
+
+void annulus_taps(float outer, float inner, std::vector<std::pair<int, int>>& out) {
+    const int rb = (int)std::ceil(outer);
+    const float or2 = outer * outer, ir2 = inner * inner;
+    for (int i = -rb; i <= rb; ++i)
+        for (int j = -rb; j <= rb; ++j) {
+            const float d = (float)(i * i + j * j);
+            if (d <= or2 && d >= ir2) out.push_back({i, j});
+        }
 }
 
 // Derived geometry and host tables (gpu_setup.rs:45-60, kernels.rs:134-205).
@@ -188,6 +219,33 @@ gt_status make_geom(const gt_params* p, bool integer_input, Geom& g) {
     g.percentile = p->percentile;
     g.characterize = p->characterize ? 1 : 0;
     g.cpu_processed = p->cpu_processed ? 1 : 0;
+    g.raw_cols = raw_cols_of(p);
+    g.col_raw = 4 + (p->cpu_processed ? 3 : 0);
+    if (g.raw_cols) {
+        if (std::ceil(std::fabs(p->sig_radius)) > 127.f ||
+            (p->has_bg_radius && std::ceil(std::fabs(p->bg_radius)) > 127.f))
+            return fail(GT_E_INVALID, "sig_radius and bg_radius must be at most 127");
+        std::vector<std::pair<int, int>> st, bt;
+        circle_taps(p->sig_radius, st);
+        if (p->has_bg_radius) {
+            // lib.rs:91: gap_radius defaults to 0 when bg_radius is given
+            const float gap = p->has_gap_radius ? p->gap_radius : 0.f;
+            annulus_taps(p->bg_radius, p->sig_radius + gap, bt);
+        }
+        if (st.size() > (size_t)RAW_MAX_TAPS || bt.size() > (size_t)RAW_MAX_TAPS)
+            return fail(GT_E_INVALID, "sig_radius/bg_radius give more than %d taps", RAW_MAX_TAPS);
+        g.raw_ns = (int32_t)st.size();
+        g.raw_nb = (int32_t)bt.size();
+        int k = 0;
+        for (const auto& t : st) {
+            g.raw_tap[k++] = (int8_t)t.first;
+            g.raw_tap[k++] = (int8_t)t.second;
+        }
+        for (const auto& t : bt) {
+            g.raw_tap[k++] = (int8_t)t.first;
+            g.raw_tap[k++] = (int8_t)t.second;
+        }
+    }
     g.disk_in_window = (g.rint <= g.a) && (g.rint <= g.kd - 1 - g.a);
     // Gaussian weights (preprocess_rows.wgsl:44): argument in binary32,
     // exponential evaluated in binary64 on the host and rounded once.
@@ -804,6 +862,10 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             if (nstreams > 1) HIP_TRY(hipEventRecord(ctx->ev_scan[si], st));
             HIP_TRY(stage(ctx, st, GT_STAGE_EMIT, [&] { return launch_emit(b, g, d_geom, st); }));
             launches += 5;
+            if (g.raw_cols) {
+                HIP_TRY(launch_raw(b, g, d_geom, st));
+                ++launches;
+            }
             if (adj) {
                 HIP_TRY(launch_adjoint(b, d_geom, ao, (uint32_t)bp.b0, st));
                 ++launches;

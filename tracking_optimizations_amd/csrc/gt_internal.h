@@ -20,6 +20,10 @@ constexpr int TILE_PX = TILE_H * TILE_W;
 // Bandpass (K1) column strip: 64 lanes x 4 columns.
 constexpr int K1_STRIP = 240;   // output columns per bandpass work item
 
+// Raw signal extraction: taps of the signal disk and of the background
+// annulus (each at most RAW_MAX_TAPS, offsets within +-127).
+constexpr int RAW_MAX_TAPS = 4096;
+
 // Percentile histogram: 4096 bins over the float32 bit pattern
 // (bin 0 = exact zero, bin b >= 1 = bits >> 19 == b - 1).
 constexpr int HIST_BINS = 4096;
@@ -76,6 +80,10 @@ struct Geom {
     int32_t ntap_gpu;      // disk taps (characterize tables)
     int32_t ntap_cpu;      // circle taps (cpu_processed tables)
     int32_t walk_in_detect;  // centroid walk fused into detect (no refine launch)
+    int32_t raw_cols;      // 0, 1 (raw_mass) or 3 (+ raw_bg_median, raw_mass_corrected)
+    int32_t col_raw;       // output column of raw_mass
+    int32_t raw_ns, raw_nb;  // signal and background tap counts
+    int8_t raw_tap[4 * RAW_MAX_TAPS];  // (dy, dx) of the signal taps, then the background taps
     float sin2_gpu[1024], cos2_gpu[1024];                // disk radius rint
     float r2_cpu[1024], sin_cpu[1024], cos_cpu[1024];    // circle radius band
 };
@@ -152,6 +160,9 @@ hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom,
                        hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+
+// Raw signal and background of every output row (gt_raw.hip).
+hipError_t launch_raw(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s);
 
 // Adjoint (gt_adjoint.hip).
 int adjoint_patch(const Geom& g);

@@ -2020,8 +2020,9 @@ hipError_t launch_frame_scan(const Batch& b, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// K5 emit: rows [frame, mass, y, x, (Rg, ecc, signal), (Rg_gpu, raw_mass_gpu,
-// signal_gpu, ecc_gpu)] in frame order, raster order of the candidate pixel
+// K5 emit: rows [frame, mass, y, x, (Rg, ecc, signal), (raw_mass, raw_bg_median,
+// raw_mass_corrected: left for k_raw), (Rg_gpu, raw_mass_gpu, signal_gpu,
+// ecc_gpu)] in frame order, raster order of the candidate pixel
 // within a frame (row positions from k_tile_scan).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ g) {
@@ -2060,6 +2061,7 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
                     o[c++] = b.rec[REC_ECC * cap + s];
                     o[c++] = b.rec[REC_SIGNAL * cap + s];
                 }
+                c += g->raw_cols;   // raw signal columns, written by k_raw
                 if (g->characterize) {
                     o[c++] = b.rec[REC_RG_GPU * cap + s];
                     o[c++] = b.rec[REC_RAW_GPU * cap + s];

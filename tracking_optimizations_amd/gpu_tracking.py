@@ -18,8 +18,9 @@ Differences, all supersets:
 ``search_range``/``memory`` link the particles frame to frame as
 execute_gpu.rs:335 does (a ``particle`` column after ``x``), and
 ``link_rust(array, search_range, memory)`` links a row table as lib.rs:202-212
-does.  Not implemented in this build (raise GtNotImplemented):
-``sig_radius``/``bg_radius`` (raw signal extraction) and the parameters the
+does.  ``sig_radius``/``bg_radius``/``gap_radius`` add the raw signal columns
+(execute_gpu.rs:301-333), computed on the device; ``bg_radius`` needs
+``sig_radius``.  Not implemented (raise GtNotImplemented): the parameters the
 reference itself rejects (maxsize, threshold, invert, topn, preprocess).
 """
 from __future__ import annotations
