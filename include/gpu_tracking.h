@@ -270,6 +270,39 @@ gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad
                          float* grad_frames, uint32_t flags);
 void gt_adjoint_free(gt_adjoint* a);
 
+/* --- movie files (gpu-tracking/src/decoderiter.rs, execute_gpu.rs:510-542) -- */
+
+typedef struct gt_file gt_file;
+
+/* Opens a TIFF (.tif/.tiff: the pages, decoderiter.rs:9-71) or ETS (.ets:
+ * the frames of `channel`, MinimalETSParser, decoderiter.rs:77-260) movie
+ * and reads its frame table; channel is ignored for TIFF, and UINT32_MAX
+ * opens an ETS file for gt_file_channels only (no frames).  No device
+ * needed.  TIFF: classic or BigTIFF, 8/16-bit unsigned grey strips,
+ * uncompressed, LZW, Deflate or PackBits, optional horizontal predictor.
+ * ETS: uint16 data. */
+gt_status gt_file_open(const char* path, uint32_t channel, gt_file** out);
+/* kind 0 TIFF, 1 ETS; frame shape h x w as stored (ETS: h = dims[1],
+ * w = dims[0], the layout parse_ets returns); dtype 0 u16, 2 u8. */
+gt_status gt_file_info(const gt_file* f, int* kind, uint64_t* n_frames, uint32_t* h,
+                       uint32_t* w, int* dtype);
+/* ETS channel numbers present in the file (TIFF: none). */
+gt_status gt_file_channels(const gt_file* f, uint32_t* channels, uint32_t cap, uint32_t* n);
+/* Decodes frames [first, first + count) into dst (host memory,
+ * count x h x w elements of the file's dtype). */
+gt_status gt_file_read(const gt_file* f, uint64_t first, uint64_t count, void* dst);
+void gt_file_close(gt_file* f);
+
+/* execute_file (execute_gpu.rs:510-542, behind batch_file_rust, lib.rs:
+ * 123-200): decode + locate a movie file.  Frames are read in chunks of
+ * about 256 MiB into pinned host buffers, the next chunk being read while
+ * the current one is located; with search_range the rows are linked once
+ * over all frames.  ETS frames are located with the ETS header's
+ * (dims[0], dims[1]) as (rows, columns), as execute_file does.  flags: 0 or
+ * GT_PLAIN. */
+gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel,
+                         const gt_params* p, uint32_t flags, gt_result** out);
+
 /* --- linking (gpu-tracking/src/linking.rs) --------------------------------- */
 
 /* Statistics of the last link on a context. */

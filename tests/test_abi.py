@@ -167,50 +167,6 @@ def test_python_api_errors_before_device():
         batch(np.zeros((2, 32, 32), np.float64), 9)
 
 
-def _write_tiff(path, pages, big_endian=False):
-    e = ">" if big_endian else "<"
-    h, w = pages[0].shape
-    bps = pages[0].dtype.itemsize * 8
-    out = bytearray((b"MM" if big_endian else b"II") + struct.pack(e + "HI", 42, 8))
-    offsets = []
-    data = []
-    for p in pages:
-        data.append(p.astype(p.dtype.newbyteorder(e)).tobytes())
-    ifd_size = 2 + 8 * 12 + 4
-    pos = 8
-    ifd_pos = []
-    for i in range(len(pages)):
-        ifd_pos.append(pos)
-        pos += ifd_size
-    for d in data:
-        offsets.append(pos)
-        pos += len(d)
-    for i, p in enumerate(pages):
-        nxt = ifd_pos[i + 1] if i + 1 < len(pages) else 0
-        tags = [(256, 4, w), (257, 4, h), (258, 3, bps), (259, 3, 1), (262, 3, 1),
-                (273, 4, offsets[i]), (277, 3, 1), (279, 4, len(data[i]))]
-        out += struct.pack(e + "H", len(tags))
-        for tag, typ, val in tags:
-            v = struct.pack(e + ("H" if typ == 3 else "I"), val).ljust(4, b"\0")
-            out += struct.pack(e + "HHI", tag, typ, 1) + v
-        out += struct.pack(e + "I", nxt)
-    for d in data:
-        out += d
-    open(path, "wb").write(bytes(out))
-
-
-@pytest.mark.parametrize("dtype,be", [(np.uint16, False), (np.uint16, True), (np.uint8, False)])
-def test_tiff_reader_roundtrip(tmp_path, dtype, be):
-    from tracking_optimizations_amd.tiff import read_tiff
-    rng = np.random.default_rng(0)
-    pages = [rng.integers(0, np.iinfo(dtype).max, (17, 23)).astype(dtype) for _ in range(3)]
-    f = tmp_path / "m.tif"
-    _write_tiff(f, pages, be)
-    got = read_tiff(str(f))
-    assert got.shape == (3, 17, 23) and got.dtype == dtype
-    assert np.array_equal(got, np.stack(pages))
-
-
 def test_batch_file_rejects_unknown_extension():
     from tracking_optimizations_amd import GtInvalid, batch_file
     with pytest.raises(GtInvalid):

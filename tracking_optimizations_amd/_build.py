@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgpu_tracking.so")
-SOURCES = ["gt_kernels.hip", "gt_synth.hip", "gt_adjoint.hip", "gt_link.hip", "gt_raw.hip", "gt_api.hip"]
+SOURCES = ["gt_kernels.hip", "gt_synth.hip", "gt_adjoint.hip", "gt_link.hip", "gt_raw.hip", "gt_ingest.hip", "gt_api.hip"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
@@ -56,7 +56,7 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         list(ex.map(lambda so: _compile(*so), zip(SOURCES, objs)))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -43,7 +43,8 @@ EXPORTS = [
     "gt_adjoint_ncols", "gt_adjoint_patch", "gt_adjoint_data", "gt_adjoint_candidates",
     "gt_adjoint_sigma", "gt_adjoint_origin", "gt_adjoint_jacobian", "gt_adjoint_vjp",
     "gt_adjoint_free", "gt_context_set_profiling", "gt_context_stage_times", "gt_link",
-    "gt_context_link_stats",
+    "gt_context_link_stats", "gt_file_open", "gt_file_info", "gt_file_channels", "gt_file_read",
+    "gt_file_close", "gt_locate_file",
 ]
 
 STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
@@ -207,6 +208,16 @@ def load() -> ctypes.CDLL:
         L.gt_link.argtypes = [vp, P(ctypes.c_float), ctypes.c_uint64, ctypes.c_uint32,
                               ctypes.c_float, ctypes.c_uint64, P(ctypes.c_uint64)]
         L.gt_context_link_stats.argtypes = [vp, P(GtLinkStats)]
+        L.gt_file_open.argtypes = [ctypes.c_char_p, ctypes.c_uint32, P(vp)]
+        L.gt_file_info.argtypes = [vp, P(ctypes.c_int), P(ctypes.c_uint64), P(ctypes.c_uint32),
+                                   P(ctypes.c_uint32), P(ctypes.c_int)]
+        L.gt_file_channels.argtypes = [vp, P(ctypes.c_uint32), ctypes.c_uint32,
+                                       P(ctypes.c_uint32)]
+        L.gt_file_read.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp]
+        L.gt_file_close.argtypes = [vp]
+        L.gt_file_close.restype = None
+        L.gt_locate_file.argtypes = [vp, ctypes.c_char_p, ctypes.c_uint32, P(GtParams),
+                                     ctypes.c_uint32, P(vp)]
         L.gt_locate_adjoint.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
                                         ctypes.c_uint32, P(GtParams), ctypes.c_uint32, P(vp)]
         L.gt_adjoint_rows.argtypes = [vp]

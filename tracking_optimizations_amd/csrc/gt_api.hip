@@ -8,7 +8,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <future>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1009,6 +1011,10 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
 
 }  // namespace
 
+namespace gt {
+gt_status set_last_error(gt_status s, const std::string& msg) { return fail(s, "%s", msg.c_str()); }
+}  // namespace gt
+
 // ---- C ABI ----------------------------------------------------------------
 
 extern "C" {
@@ -1342,6 +1348,100 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
     for (uint64_t i = 0; i < n_frames; ++i)
         v.push_back({frames[i].data, frames[i].h, frames[i].w, frames[i].frame_index});
     return run_locate(ctx, v, in_type, p, flags, out);
+}
+
+gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel, const gt_params* p,
+                         uint32_t flags, gt_result** out) {
+    if (!out) return fail(GT_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (flags & (GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE))
+        return fail(GT_E_INVALID, "gt_locate_file reads frames into host memory and returns host rows");
+    GT_TRY(check_params(p, 0, 0));
+    gt_file* fp = nullptr;
+    GT_TRY(gt_file_open(path, channel, &fp));
+    std::unique_ptr<gt_file, void (*)(gt_file*)> file(fp, gt_file_close);
+    int kind = 0, dtype = 0;
+    uint64_t n = 0;
+    uint32_t h = 0, w = 0;
+    GT_TRY(gt_file_info(fp, &kind, &n, &h, &w, &dtype));
+    // execute_file (execute_gpu.rs:525-531) hands ETS frames on with the
+    // ETS header's (dims[0], dims[1]) as (rows, columns)
+    if (kind == 1) std::swap(h, w);
+    GT_TRY(check_params(p, h, w));
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int in_type = dtype == 0 ? IN_U16 : IN_U8;
+    const size_t fb = (size_t)h * w * (dtype == 0 ? 2 : 1);
+    // frames are read in chunks of about 256 MiB into two pinned buffers:
+    // the next chunk is read while the current one is located
+    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, ((size_t)256 << 20) / fb));
+    struct Pinned {
+        void* p = nullptr;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    } buf[2];
+    if (n) {
+        for (auto& b : buf) HIP_TRY(hipHostMalloc(&b.p, chunk * fb, hipHostMallocDefault));
+    }
+    gt_params q = *p;
+    q.has_search_range = 0;   // linked once over all frames below
+    std::unique_ptr<gt_result> res(new gt_result());
+    {
+        const char* names[16];
+        const char* dtypes[16];
+        uint32_t nc = 0;
+        GT_TRY(gt_column_names(&q, names, dtypes, 16, &nc));
+        res->ncols = nc;
+    }
+    std::vector<FrameIn> all;
+    all.reserve(n);
+    for (uint64_t i = 0; i < n; ++i) all.push_back({nullptr, h, w, (int64_t)i});
+    gt_timing total{};
+    auto read_chunk = [&](uint64_t c) -> gt_status {
+        const uint64_t a = c * chunk, m = std::min<uint64_t>(chunk, n - a);
+        return gt_file_read(fp, a, m, buf[c & 1].p);
+    };
+    const uint64_t nchunks = n ? (n + chunk - 1) / chunk : 0;
+    if (nchunks) GT_TRY(read_chunk(0));
+    for (uint64_t c = 0; c < nchunks; ++c) {
+        std::future<gt_status> next;
+        if (c + 1 < nchunks) next = std::async(std::launch::async, read_chunk, c + 1);
+        const uint64_t a = c * chunk, m = std::min<uint64_t>(chunk, n - a);
+        std::vector<FrameIn> fr(all.begin() + a, all.begin() + a + m);
+        for (uint64_t i = 0; i < m; ++i) fr[i].ptr = (const char*)buf[c & 1].p + i * fb;
+        gt_result* part = nullptr;
+        const gt_status st = run_locate(ctx, fr, in_type, &q, flags & GT_PLAIN, &part);
+        const gt_status rs = next.valid() ? next.get() : GT_OK;
+        if (st != GT_OK) return st;
+        if (rs != GT_OK) {
+            delete part;
+            return rs;
+        }
+        res->data.insert(res->data.end(), part->data.begin(), part->data.end());
+        res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
+        res->rows += part->rows;
+        delete part;
+        total.device_ms += ctx->last.device_ms;
+        total.total_ms += ctx->last.total_ms;
+        total.frames += ctx->last.frames;
+        total.pixels += ctx->last.pixels;
+        total.candidates += ctx->last.candidates;
+        total.rows += ctx->last.rows;
+        total.launches += ctx->last.launches;
+    }
+    if (p->has_search_range) GT_TRY(link_result(ctx, all, p, res.get()));
+    {
+        const char* names[16];
+        const char* dtypes[16];
+        uint32_t nc = 0;
+        GT_TRY(gt_column_names(p, names, dtypes, 16, &nc));
+        res->names.assign(names, names + nc);
+        res->dtypes.assign(dtypes, dtypes + nc);
+    }
+    ctx->last = total;
+    *out = res.release();
+    return GT_OK;
 }
 
 gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,

@@ -35,7 +35,8 @@ from ._lib import (GT_FRAMES_ON_DEVICE, GT_PLAIN, GtError, GtInvalid, GtNotImple
                    GtParams, Result, check, default_context, load)
 
 __all__ = ["batch", "batch_rust", "batch_file", "batch_file_rust", "make_params",
-           "locate", "column_names", "link_rust", "GtError", "GtInvalid", "GtNotImplemented"]
+           "locate", "column_names", "link_rust", "parse_ets", "read_movie", "MovieFile",
+           "GtError", "GtInvalid", "GtNotImplemented"]
 
 
 def make_params(diameter, minmass=None, maxsize=None, separation=None, noise_size=None,
@@ -201,24 +202,86 @@ def batch(video, diameter, minmass=None, maxsize=None, separation=None, noise_si
     return pd.DataFrame(arr, columns=columns).astype(columns)
 
 
+class MovieFile:
+    """A TIFF or ETS movie opened by the library's reader (gt_file_open)."""
+
+    def __init__(self, path, channel: int = 0):
+        self._h = ctypes.c_void_p()
+        check(load().gt_file_open(os.fsencode(path), channel, ctypes.byref(self._h)))
+        kind, n = ctypes.c_int(), ctypes.c_uint64()
+        h, w, dt = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int()
+        check(load().gt_file_info(self._h, ctypes.byref(kind), ctypes.byref(n), ctypes.byref(h),
+                                  ctypes.byref(w), ctypes.byref(dt)))
+        self.kind = "tiff" if kind.value == 0 else "ets"
+        self.n_frames, self.shape = n.value, (h.value, w.value)
+        self.dtype = np.dtype(np.uint16 if dt.value == 0 else np.uint8)
+
+    def channels(self) -> list[int]:
+        n = ctypes.c_uint32()
+        check(load().gt_file_channels(self._h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_uint32 * max(n.value, 1))()
+        check(load().gt_file_channels(self._h, buf, n.value, ctypes.byref(n)))
+        return [int(buf[i]) for i in range(n.value)]
+
+    def read(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        count = self.n_frames - first if count is None else count
+        out = np.empty((count, *self.shape), self.dtype)
+        check(load().gt_file_read(self._h, first, count, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            load().gt_file_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_movie(path, channel: int = 0) -> np.ndarray:
+    """Every frame of a TIFF (pages) or ETS (frames of `channel`) movie as an
+    (n, h, w) uint16/uint8 array (decoderiter.rs)."""
+    with MovieFile(path, channel) as f:
+        return f.read()
+
+
+def parse_ets(path) -> dict:
+    """lib.rs:214-232: {channel: uint16 array (n_frames, dims[1], dims[0])}."""
+    with MovieFile(path, 0xFFFFFFFF) as f:
+        chans = f.channels()
+    return {c: read_movie(path, c) for c in chans}
+
+
 def batch_file_rust(filename, diameter, channel=None, minmass=None, maxsize=None, separation=None,
                     noise_size=None, smoothing_size=None, threshold=None, invert=None,
                     percentile=None, topn=None, preprocess=None, max_iterations=None,
                     characterize=None, filter_close=None, search_range=None, memory=None,
-                    cpu_processed=None, sig_radius=None, bg_radius=None, gap_radius=None):
-    """lib.rs:123-200 for TIFF files (execute_gpu.rs:510-542)."""
-    from .tiff import read_tiff
-    ext = os.path.splitext(filename)[1].lower().lstrip(".")
-    if ext in ("tif", "tiff"):
-        video = read_tiff(filename)
-    elif ext == "ets":
-        raise GtNotImplemented(_lib.GT_E_NOT_IMPLEMENTED, "ets files are not supported in this build")
-    else:
-        raise GtInvalid(_lib.GT_E_INVALID, f"File extension '{ext}' not supported")
-    return batch_rust(video, diameter, minmass, maxsize, separation, noise_size, smoothing_size,
-                      threshold, invert, percentile, topn, preprocess, max_iterations,
-                      characterize, filter_close, search_range, memory, cpu_processed, sig_radius,
-                      bg_radius, gap_radius)
+                    cpu_processed=None, sig_radius=None, bg_radius=None, gap_radius=None,
+                    device: int = 0, ctx=None):
+    """lib.rs:123-200 -> execute_file (execute_gpu.rs:510-542): decode and
+    locate a TIFF or ETS movie; returns (float32 array [N, C], columns)."""
+    p = make_params(diameter, minmass, maxsize, separation, noise_size, smoothing_size, threshold,
+                    invert, percentile, topn, preprocess, max_iterations, characterize,
+                    filter_close, search_range, memory, cpu_processed, sig_radius, bg_radius,
+                    gap_radius)
+    check(load().gt_params_validate(ctypes.byref(p), 0, 0))
+    path = os.fsencode(filename)
+    ch = 0 if channel is None else int(channel)
+    MovieFile(filename, ch).close()   # reports unreadable files before a device is needed
+    ctx = ctx or default_context(device)
+    out = ctypes.c_void_p()
+    check(load().gt_locate_file(ctx.handle, path, ch, ctypes.byref(p), 0, ctypes.byref(out)))
+    res = Result(out)
+    return res.rows(), res.columns()
 
 
 def batch_file(path, diameter, channel=None, minmass=None, maxsize=None, separation=None,
