@@ -85,10 +85,12 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
-    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link"], default="movie",
+    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link", "file"],
+                    default="movie",
                     help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames; "
                          "adjoint: 256 x 512^2 localisation errors and Jacobians; "
-                         "link: locate + link, 2000 x 1024^2, 3000 blinking particles")
+                         "link: locate + link, 2000 x 1024^2, 3000 blinking particles; "
+                         "file: batch_file on a 256 x 2048^2 uint16 TIFF")
     ap.add_argument("--link-cpu-frames", type=int, default=150,
                     help="frames of the link workload timed with the CPU oracle linker")
     args = ap.parse_args()
@@ -107,6 +109,8 @@ def main() -> int:
         return bench_adjoint(args)
     if args.workload == "link":
         return bench_link(args)
+    if args.workload == "file":
+        return bench_file(args)
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
     if args.no_percentile:
@@ -520,6 +524,81 @@ def bench_link(args) -> int:
                       "link_rust_only_frames_per_s": nf / link_el,
                       "cpu_baseline": cpu}), flush=True)
     movie.free()
+    return 0
+
+
+def _write_tiff_u16(path: str, frames: np.ndarray) -> None:
+    """Uncompressed little-endian TIFF, one strip per page."""
+    import struct
+    n, h, w = frames.shape
+    ifd = 2 + 8 * 12 + 4
+    data0 = 8 + n * ifd
+    page = h * w * 2
+    with open(path, "wb") as f:
+        f.write(b"II" + struct.pack("<HI", 42, 8))
+        for i in range(n):
+            nxt = 8 + (i + 1) * ifd if i + 1 < n else 0
+            tags = [(256, 4, w), (257, 4, h), (258, 3, 16), (259, 3, 1), (262, 3, 1),
+                    (273, 4, data0 + i * page), (277, 3, 1), (279, 4, page)]
+            f.write(struct.pack("<H", len(tags)))
+            for tag, typ, val in tags:
+                f.write(struct.pack("<HHI", tag, typ, 1) +
+                        struct.pack("<" + ("H" if typ == 3 else "I"), val).ljust(4, b"\0"))
+            f.write(struct.pack("<I", nxt))
+        for i in range(n):
+            f.write(np.ascontiguousarray(frames[i], "<u2").tobytes())
+
+
+def bench_file(args) -> int:
+    """batch_file end to end on a 256-frame 2048x2048 uint16 TIFF (the
+    headline movie's first frames, uncompressed, in the page cache after the
+    warm-up): native decode into pinned buffers, upload and locate, rows
+    back on the host.  Beside it: batch on the same frames from a pageable
+    host array (staged through the same pinned buffers), and the reader alone."""
+    import tempfile
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import (MovieFile, batch_file_rust, locate,
+                                                         make_params)
+    ctx = _lib.Context(0)
+    nf = 256 if args.frames == 1000 else args.frames
+    spec = synth.movie_2048(seed=2, n_frames=nf)
+    video = synth.generate_host(ctx, spec)
+    kw = dict(separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "movie.tif")
+        _write_tiff_u16(path, video)
+        log(f"wrote {os.path.getsize(path) / 1e9:.2f} GB TIFF")
+        for _ in range(max(1, args.warmup)):
+            rows, _ = batch_file_rust(path, 11, ctx=ctx, **kw)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rows, _ = batch_file_rust(path, 11, ctx=ctx, **kw)
+        el = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        with MovieFile(path) as m:
+            buf = m.read()
+        rd = time.perf_counter() - t0
+        del buf
+    params = make_params(11, **kw)
+    ref = locate(video, params, ctx=ctx)[0]
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ref = locate(video, params, ctx=ctx)[0]
+    hel = (time.perf_counter() - t0) / args.steps
+    same = bool(np.array_equal(rows, ref))
+    log(f"batch_file: {nf / el:.0f} frames/s ({el * 1e3:.0f} ms); batch from a host array: "
+        f"{nf / hel:.0f} frames/s; reader alone {nf / rd:.0f} frames/s "
+        f"({video.nbytes / rd / 1e9:.1f} GB/s); rows identical: {same}")
+    print(json.dumps({"metric": "frames/s, batch_file (TIFF decode + upload + locate), "
+                                "256 x 2048^2 uint16",
+                      "value": nf / el, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": el * 1e3, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic movie written as an uncompressed TIFF (page cache)",
+                      "config": {"workload": f"{nf} x 2048x2048 uint16 TIFF", "params": kw,
+                                 "rows": int(rows.shape[0])},
+                      "host_array_frames_per_s": nf / hel, "reader_frames_per_s": nf / rd,
+                      "rows_identical_to_batch": same}), flush=True)
     return 0
 
 

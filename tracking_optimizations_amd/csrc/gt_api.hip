@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <future>
 #include <map>
 #include <memory>
@@ -350,6 +351,9 @@ struct gt_context {
     DevBuf geom, frames_tab, bases, rows, rows_cand, counters;
     Workspace ws[kStreams];
     LinkStats last_link;
+    // pinned host buffers of gt_locate_file (two chunks), kept for reuse
+    void* pinned[2] = {nullptr, nullptr};
+    size_t pinned_bytes = 0;
 };
 
 struct gt_result {
@@ -1150,6 +1154,8 @@ gt_status gt_context_destroy(gt_context* ctx) {
                       &ctx->counters};
     for (DevBuf* b : bufs) b->release();
     for (int i = 0; i < kStreams; ++i) ctx->ws[i].release();
+    for (void* b : ctx->pinned)
+        if (b) (void)hipHostFree(b);
     collect_stage_times(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
@@ -1306,6 +1312,125 @@ gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch) {
     return GT_OK;
 }
 
+// Host-resident frames, located chunk by chunk (about 256 MiB each) through
+// the context's two pinned buffers: while chunk c is uploaded and located,
+// up to 8 threads fill chunk c + 1 (fill(frame, dst) copies or decodes one
+// frame).  With search_range the rows are linked once over all frames.
+static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
+                               const gt_params* p, uint32_t flags,
+                               const std::function<gt_status(size_t, void*)>& fill,
+                               gt_result** out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t es = elem_size(in_type);
+    const size_t limit = (size_t)256 << 20;
+    std::vector<size_t> starts(1, 0), offs(frames.size());
+    size_t cur = 0, max_chunk = 0;
+    for (size_t i = 0; i < frames.size(); ++i) {
+        const size_t fb = al256((size_t)frames[i].h * frames[i].w * es);
+        if (cur > 0 && cur + fb > limit) {
+            starts.push_back(i);
+            max_chunk = std::max(max_chunk, cur);
+            cur = 0;
+        }
+        offs[i] = cur;
+        cur += fb;
+    }
+    max_chunk = std::max(max_chunk, cur);
+    starts.push_back(frames.size());
+    const size_t nchunks = frames.empty() ? 0 : starts.size() - 1;
+    if (nchunks && ctx->pinned_bytes < max_chunk) {
+        for (auto& b : ctx->pinned) {
+            if (b) (void)hipHostFree(b);
+            b = nullptr;
+        }
+        ctx->pinned_bytes = 0;
+        for (auto& b : ctx->pinned) HIP_TRY(hipHostMalloc(&b, max_chunk, hipHostMallocDefault));
+        ctx->pinned_bytes = max_chunk;
+    }
+    char* const buf[2] = {(char*)ctx->pinned[0], (char*)ctx->pinned[1]};
+    // (error messages are per thread: the filler's travels back with it)
+    using Status = std::pair<gt_status, std::string>;
+    auto fill_chunk = [&](size_t c) -> Status {
+        const size_t a = starts[c], m = starts[c + 1] - a;
+        const size_t nt = std::min<size_t>(8, m), per = (m + nt - 1) / nt;
+        std::vector<std::future<Status>> parts;
+        for (size_t t = 0; t < nt; ++t) {
+            const size_t f0 = a + t * per, f1 = std::min(a + m, f0 + per);
+            if (f0 >= f1) break;
+            parts.push_back(std::async(std::launch::async, [&, f0, f1, c]() -> Status {
+                for (size_t fi = f0; fi < f1; ++fi) {
+                    const gt_status r = fill(fi, buf[c & 1] + offs[fi]);
+                    if (r != GT_OK) return {r, gt_last_error()};
+                }
+                return {GT_OK, {}};
+            }));
+        }
+        Status st{GT_OK, {}};
+        for (auto& q : parts) {
+            Status r = q.get();
+            if (st.first == GT_OK) st = std::move(r);
+        }
+        return st;
+    };
+    gt_params q = *p;
+    q.has_search_range = 0;   // linked once over all frames below
+    std::unique_ptr<gt_result> res(new gt_result());
+    {
+        const char* names[16];
+        const char* dtypes[16];
+        uint32_t nc = 0;
+        GT_TRY(gt_column_names(&q, names, dtypes, 16, &nc));
+        res->ncols = nc;
+    }
+    gt_timing total{};
+    if (nchunks) {
+        const Status r0 = fill_chunk(0);
+        if (r0.first != GT_OK) return fail(r0.first, "%s", r0.second.c_str());
+    }
+    for (size_t c = 0; c < nchunks; ++c) {
+        std::future<Status> next;
+        if (c + 1 < nchunks) next = std::async(std::launch::async, fill_chunk, c + 1);
+        std::vector<FrameIn> fr(frames.begin() + starts[c], frames.begin() + starts[c + 1]);
+        for (size_t i = 0; i < fr.size(); ++i) fr[i].ptr = buf[c & 1] + offs[starts[c] + i];
+        gt_result* part = nullptr;
+        const gt_status st = run_locate(ctx, fr, in_type, &q, flags, &part);
+        const Status rs = next.valid() ? next.get() : Status{GT_OK, {}};
+        if (st != GT_OK) return st;
+        std::unique_ptr<gt_result> partp(part);
+        if (rs.first != GT_OK) return fail(rs.first, "%s", rs.second.c_str());
+        res->data.insert(res->data.end(), part->data.begin(), part->data.end());
+        res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
+        res->rows += part->rows;
+        total.device_ms += ctx->last.device_ms;
+        total.frames += ctx->last.frames;
+        total.pixels += ctx->last.pixels;
+        total.candidates += ctx->last.candidates;
+        total.rows += ctx->last.rows;
+        total.launches += ctx->last.launches;
+    }
+    if (p->has_search_range) GT_TRY(link_result(ctx, frames, p, res.get()));
+    {
+        const char* names[16];
+        const char* dtypes[16];
+        uint32_t nc = 0;
+        GT_TRY(gt_column_names(p, names, dtypes, 16, &nc));
+        res->names.assign(names, names + nc);
+        res->dtypes.assign(dtypes, dtypes + nc);
+    }
+    total.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ctx->last = total;
+    *out = res.release();
+    return GT_OK;
+}
+
+// Host frames go through the pinned staging unless the call asks for the
+// per-frame path, device-resident rows or an adjoint.
+static bool staged_path(uint32_t flags) {
+    return !(flags & (GT_FRAMES_ON_DEVICE | GT_PLAIN | GT_RESULT_ON_DEVICE));
+}
+
 static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n, uint32_t h,
                                 uint32_t w, int in_type, const gt_params* p, uint32_t flags,
                                 gt_result** out) {
@@ -1316,6 +1441,13 @@ static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n,
     for (uint64_t i = 0; i < n; ++i) v.push_back({(const char*)frames + i * fb, h, w, (int64_t)i});
     if (n == 0) {
         GT_TRY(check_params(p, h, w));
+    }
+    if (staged_path(flags) && n > 0 && ctx && out) {
+        GT_TRY(check_params(p, h, w));
+        return locate_staged(ctx, v, in_type, p, flags, [&](size_t fi, void* dst) {
+            std::memcpy(dst, v[fi].ptr, fb);
+            return GT_OK;
+        }, out);
     }
     return run_locate(ctx, v, in_type, p, flags, out);
 }
@@ -1347,6 +1479,21 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
     v.reserve(n_frames);
     for (uint64_t i = 0; i < n_frames; ++i)
         v.push_back({frames[i].data, frames[i].h, frames[i].w, frames[i].frame_index});
+    if (staged_path(flags) && n_frames > 0 && ctx && out) {
+        for (const auto& f : v) {
+            GT_TRY(check_params(p, f.h, f.w));
+            if (!f.ptr) return fail(GT_E_INVALID, "frame pointer is NULL");
+        }
+        if (p->has_search_range)
+            for (size_t f = 1; f < v.size(); ++f)
+                if ((float)v[f].fi == (float)v[f - 1].fi)
+                    return fail(GT_E_INVALID, "search_range needs consecutive frames with distinct indices");
+        const size_t es = elem_size(in_type);
+        return locate_staged(ctx, v, in_type, p, flags, [&](size_t fi, void* dst) {
+            std::memcpy(dst, v[fi].ptr, (size_t)v[fi].h * v[fi].w * es);
+            return GT_OK;
+        }, out);
+    }
     return run_locate(ctx, v, in_type, p, flags, out);
 }
 
@@ -1355,8 +1502,8 @@ gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel, co
     if (!out) return fail(GT_E_INVALID, "out is NULL");
     *out = nullptr;
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
-    if (flags & (GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE))
-        return fail(GT_E_INVALID, "gt_locate_file reads frames into host memory and returns host rows");
+    if (flags & ~(uint32_t)GT_PLAIN)
+        return fail(GT_E_INVALID, "gt_locate_file takes flags 0 or GT_PLAIN");
     GT_TRY(check_params(p, 0, 0));
     gt_file* fp = nullptr;
     GT_TRY(gt_file_open(path, channel, &fp));
@@ -1369,79 +1516,11 @@ gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel, co
     // ETS header's (dims[0], dims[1]) as (rows, columns)
     if (kind == 1) std::swap(h, w);
     GT_TRY(check_params(p, h, w));
-    HIP_TRY(hipSetDevice(ctx->device));
-    const int in_type = dtype == 0 ? IN_U16 : IN_U8;
-    const size_t fb = (size_t)h * w * (dtype == 0 ? 2 : 1);
-    // frames are read in chunks of about 256 MiB into two pinned buffers:
-    // the next chunk is read while the current one is located
-    const uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(n, ((size_t)256 << 20) / fb));
-    struct Pinned {
-        void* p = nullptr;
-        ~Pinned() {
-            if (p) (void)hipHostFree(p);
-        }
-    } buf[2];
-    if (n) {
-        for (auto& b : buf) HIP_TRY(hipHostMalloc(&b.p, chunk * fb, hipHostMallocDefault));
-    }
-    gt_params q = *p;
-    q.has_search_range = 0;   // linked once over all frames below
-    std::unique_ptr<gt_result> res(new gt_result());
-    {
-        const char* names[16];
-        const char* dtypes[16];
-        uint32_t nc = 0;
-        GT_TRY(gt_column_names(&q, names, dtypes, 16, &nc));
-        res->ncols = nc;
-    }
     std::vector<FrameIn> all;
     all.reserve(n);
     for (uint64_t i = 0; i < n; ++i) all.push_back({nullptr, h, w, (int64_t)i});
-    gt_timing total{};
-    auto read_chunk = [&](uint64_t c) -> gt_status {
-        const uint64_t a = c * chunk, m = std::min<uint64_t>(chunk, n - a);
-        return gt_file_read(fp, a, m, buf[c & 1].p);
-    };
-    const uint64_t nchunks = n ? (n + chunk - 1) / chunk : 0;
-    if (nchunks) GT_TRY(read_chunk(0));
-    for (uint64_t c = 0; c < nchunks; ++c) {
-        std::future<gt_status> next;
-        if (c + 1 < nchunks) next = std::async(std::launch::async, read_chunk, c + 1);
-        const uint64_t a = c * chunk, m = std::min<uint64_t>(chunk, n - a);
-        std::vector<FrameIn> fr(all.begin() + a, all.begin() + a + m);
-        for (uint64_t i = 0; i < m; ++i) fr[i].ptr = (const char*)buf[c & 1].p + i * fb;
-        gt_result* part = nullptr;
-        const gt_status st = run_locate(ctx, fr, in_type, &q, flags & GT_PLAIN, &part);
-        const gt_status rs = next.valid() ? next.get() : GT_OK;
-        if (st != GT_OK) return st;
-        if (rs != GT_OK) {
-            delete part;
-            return rs;
-        }
-        res->data.insert(res->data.end(), part->data.begin(), part->data.end());
-        res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
-        res->rows += part->rows;
-        delete part;
-        total.device_ms += ctx->last.device_ms;
-        total.total_ms += ctx->last.total_ms;
-        total.frames += ctx->last.frames;
-        total.pixels += ctx->last.pixels;
-        total.candidates += ctx->last.candidates;
-        total.rows += ctx->last.rows;
-        total.launches += ctx->last.launches;
-    }
-    if (p->has_search_range) GT_TRY(link_result(ctx, all, p, res.get()));
-    {
-        const char* names[16];
-        const char* dtypes[16];
-        uint32_t nc = 0;
-        GT_TRY(gt_column_names(p, names, dtypes, 16, &nc));
-        res->names.assign(names, names + nc);
-        res->dtypes.assign(dtypes, dtypes + nc);
-    }
-    ctx->last = total;
-    *out = res.release();
-    return GT_OK;
+    return locate_staged(ctx, all, dtype == 0 ? IN_U16 : IN_U8, p, flags,
+                         [fp](size_t fi, void* dst) { return gt_file_read(fp, fi, 1, dst); }, out);
 }
 
 gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
