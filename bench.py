@@ -10,8 +10,10 @@ batched path's frames/s with frames and result rows resident in HBM; the plain
 per-frame path (launch + completion wait per frame, as the reference's
 execute_gpu loop does) is timed on the same movie beside it.  The roofline
 entry is for the dominant kernel, measured with HIP events on the stream the
-library launches it on; the CPU baseline is the oracle (the CPU restatement of
-the reference pipeline) on a bounded sample of the same frames.
+library launches it on, in one extra step that runs the batches on one stream
+(so that a kernel's duration is not stretched by the next batch's kernels);
+the CPU baseline is the oracle (the CPU restatement of the reference
+pipeline) on a bounded sample of the same frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -79,6 +81,8 @@ def main() -> int:
                     help="rows per bandpass work item (default: the library's 128)")
     ap.add_argument("--separate-refine", action="store_true",
                     help="walk in a separate refine kernel instead of inside detect")
+    ap.add_argument("--graph-replay", action="store_true",
+                    help="also time the steps replaying a captured HIP graph")
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-shape autotuner (rows per bandpass item, frames per "
                          "launch) before timing")
@@ -170,17 +174,13 @@ def main() -> int:
         log(f"warmup {i}: device {t['device_ms']:.1f} ms, {t['candidates']} candidates, "
             f"{t['rows']} rows")
     sync()
-    # timed region: per-stage HIP events are recorded on the stream each
-    # launch goes to, so kernel durations come from the timed steps themselves
-    ctx.set_profiling(True)
+    # timed region: K steps, frames and rows resident in HBM, two streams
     t0 = time.perf_counter()
     stats = []
     for i in range(args.steps):
         stats.append(run(batched_flags))
     sync()
     elapsed = time.perf_counter() - t0
-    stages = ctx.stage_times()
-    ctx.set_profiling(False)
     ms_per_step = 1000.0 * elapsed / args.steps
     fps = nf * args.steps / elapsed
     dev_ms = float(np.mean([s["device_ms"] for s in stats]))
@@ -188,26 +188,29 @@ def main() -> int:
     n_rows = stats[-1]["rows"]
     log(f"batched: {fps:.0f} frames/s ({ms_per_step:.1f} ms/step, device {dev_ms:.1f} ms), "
         f"{n_cand / nf:.0f} candidates/frame, {n_rows / nf:.0f} rows/frame")
-    stages = {k: (ms / args.steps, n // args.steps) for k, (ms, n) in stages.items()}
-    # the same steps without stage events, to show what the events cost
-    sync()
+    # one more step with per-stage HIP events recorded by the library on the
+    # stream each launch goes to; profiling runs the batches on one stream,
+    # so every kernel's duration is its own (no overlap with the next batch)
+    ctx.set_profiling(True)
     t1 = time.perf_counter()
-    for i in range(args.steps):
-        run(batched_flags)
+    run(batched_flags)
     sync()
-    fps_noev = nf * args.steps / (time.perf_counter() - t1)
-    log(f"batched without stage events: {fps_noev:.0f} frames/s")
-    # the same steps replaying the captured launch sequence
-    ctx.set_graphs(True)
-    run(batched_flags)  # capture
-    sync()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        run(batched_flags)
-    sync()
-    fps_graph = nf * args.steps / (time.perf_counter() - t1)
-    ctx.set_graphs(False)
-    log(f"batched, graph replay: {fps_graph:.0f} frames/s")
+    fps_prof = nf / (time.perf_counter() - t1)
+    stages = ctx.stage_times()
+    ctx.set_profiling(False)
+    log(f"profiled step (one stream, stage events): {fps_prof:.0f} frames/s")
+    fps_graph = None
+    if args.graph_replay:
+        ctx.set_graphs(True)
+        run(batched_flags)  # capture
+        sync()
+        t1 = time.perf_counter()
+        for i in range(args.steps):
+            run(batched_flags)
+        sync()
+        fps_graph = nf * args.steps / (time.perf_counter() - t1)
+        ctx.set_graphs(False)
+        log(f"batched, graph replay: {fps_graph:.0f} frames/s")
     for k, (ms, n) in stages.items():
         if n:
             log(f"  stage {k:9s}: {ms:8.3f} ms/step over {n} launches ({ms / n:.4f} ms/launch)")
@@ -295,7 +298,7 @@ def main() -> int:
                      "candidates_per_frame": n_cand / nf, "rows_per_frame": n_rows / nf,
                      "device_ms_per_step": dev_ms,
                      "stage_ms": {k: v[0] for k, v in stages.items() if v[1]}},
-        "frames_per_s_without_stage_events": fps_noev,
+        "frames_per_s_profiled_one_stream": fps_prof,
         "frames_per_s_graph_replay": fps_graph,
         "tuning": tuning,
         "plain": plain,

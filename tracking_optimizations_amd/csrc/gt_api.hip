@@ -656,7 +656,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     ctx->last.pixels = total_px;
 
     // ---- workspaces -------------------------------------------------------
-    const int nstreams = (plain || plan.size() < 2) ? 1 : kStreams;
+    const int nstreams = (plain || plan.size() < 2 || ctx->profiling) ? 1 : kStreams;
     const uint64_t slot_cap = maxneed.tiles * TILE_PX;
     const size_t nfb = maxneed.nframes;
     // per-frame: hist (4096 u32), frame_rows, frame_cands, spare, selstate(4), thr(double), row_base(u64)
