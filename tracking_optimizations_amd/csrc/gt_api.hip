@@ -351,6 +351,7 @@ struct gt_context {
     DevBuf geom, frames_tab, bases, rows, rows_cand, counters;
     Workspace ws[kStreams];
     LinkStats last_link;
+    LinkCache* link_cache = nullptr;   // linker buffers, created on first use
     // pinned host buffers of gt_locate_file (two chunks), kept for reuse
     void* pinned[2] = {nullptr, nullptr};
     size_t pinned_bytes = 0;
@@ -523,7 +524,8 @@ gt_status link_result(gt_context* ctx, const std::vector<FrameIn>& frames, const
     std::vector<uint64_t> ids(n);
     const uint32_t mem = p->has_memory ? (uint32_t)std::min<uint64_t>(p->memory, UINT32_MAX) : 0;
     std::string err;
-    const hipError_t e = link_steps(ctx->streams[0], ys.data(), xs.data(), step_off,
+    if (!ctx->link_cache) ctx->link_cache = link_cache_create();
+    const hipError_t e = link_steps(ctx->streams[0], ctx->link_cache, ys.data(), xs.data(), step_off,
                                     p->search_range, mem, ids.data(), &ctx->last_link, err);
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? GT_E_OOM
@@ -1156,6 +1158,7 @@ gt_status gt_context_destroy(gt_context* ctx) {
     for (int i = 0; i < kStreams; ++i) ctx->ws[i].release();
     for (void* b : ctx->pinned)
         if (b) (void)hipHostFree(b);
+    if (ctx->link_cache) link_cache_destroy(ctx->link_cache);
     collect_stage_times(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->ev0);
@@ -1764,7 +1767,8 @@ gt_status gt_link(gt_context* ctx, const float* rows, uint64_t n_rows, uint32_t 
     if (ncols < 4) return fail(GT_E_INVALID, "rows need at least 4 columns (frame, mass, y, x)");
     HIP_TRY(hipSetDevice(ctx->device));
     std::string err;
-    const hipError_t e = link_rows(ctx->streams[0], rows, n_rows, ncols, search_range,
+    if (!ctx->link_cache) ctx->link_cache = link_cache_create();
+    const hipError_t e = link_rows(ctx->streams[0], ctx->link_cache, rows, n_rows, ncols, search_range,
                                    (uint32_t)std::min<uint64_t>(memory, UINT32_MAX), ids,
                                    &ctx->last_link, err);
     if (e != hipSuccess)

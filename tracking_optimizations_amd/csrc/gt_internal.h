@@ -181,13 +181,17 @@ struct LinkStats {
 // Particle ids of the n = step_off.back() positions (ys, xs: host arrays),
 // step s being positions [step_off[s], step_off[s + 1]).  On failure `err`
 // says why.
-hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
+// Device buffers and pinned tables kept between calls (one per context).
+struct LinkCache;
+LinkCache* link_cache_create();
+void link_cache_destroy(LinkCache* c);
+hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const float* xs,
                       const std::vector<uint32_t>& step_off, float search_range,
                       uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err);
 // link_rust: steps are FrameSubsetter runs of column 0, y = column 2, x = 3.
-hipError_t link_rows(hipStream_t st, const float* rows, uint64_t n_rows, uint32_t ncols,
-                     float search_range, uint32_t memory, uint64_t* ids, LinkStats* stats,
-                     std::string& err);
+hipError_t link_rows(hipStream_t st, LinkCache* cache, const float* rows, uint64_t n_rows,
+                     uint32_t ncols, float search_range, uint32_t memory, uint64_t* ids,
+                     LinkStats* stats, std::string& err);
 
 // Synthetic frames (gt_synth.hip).
 hipError_t launch_synth(uint16_t* frames, uint64_t nframes, uint32_t h,

@@ -169,21 +169,40 @@ struct Assign {
     }
 };
 
-struct DevArena {
-    std::vector<void*> ptrs;
-    ~DevArena() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <class T> hipError_t alloc(T** p, size_t count) {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, std::max<size_t>(count * sizeof(T), 256));
-        if (e == hipSuccess) ptrs.push_back(q);
-        *p = (T*)q;
+template <bool PINNED>
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n) {
+        n = std::max<size_t>(n, 256);
+        if (p && n <= bytes) return hipSuccess;
+        release();
+        n = (n + 4095) & ~(size_t)4095;
+        const hipError_t e = PINNED ? hipHostMalloc(&p, n, hipHostMallocDefault) : hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
         return e;
     }
+    void release() {
+        if (p) (void)(PINNED ? hipHostFree(p) : hipFree(p));
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+    ~Buf() { release(); }
 };
 
 }  // namespace
+
+// Device buffers and pinned host tables of the linker, kept by the context
+// between calls (grown, never shrunk).
+struct LinkCache {
+    Buf<false> y, x, stepof, stepoff, k0, v0, k1, v1, cnt, off, cd, dist, sort_tmp, scan_tmp;
+    Buf<true> off_h, cd_h, dist_h;
+};
+
+LinkCache* link_cache_create() { return new LinkCache(); }
+void link_cache_destroy(LinkCache* c) { delete c; }
 
 #define LK_TRY(x)                                                 \
     do {                                                          \
@@ -194,7 +213,7 @@ struct DevArena {
         }                                                         \
     } while (0)
 
-hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
+hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const float* xs,
                       const std::vector<uint32_t>& step_off, float search_range,
                       uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err) {
     const uint32_t nsteps = (uint32_t)step_off.size() - 1;
@@ -214,27 +233,31 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
     const auto t0 = std::chrono::steady_clock::now();
 
     // ---- candidates on the device --------------------------------------------
-    std::vector<uint32_t> off_h(nt + 1, 0);
-    std::vector<uint32_t> cd_h;
-    std::vector<float> cdist_h;
+    LinkCache& C = *cache;
+    LK_TRY(C.off_h.ensure((nt + 1) * 4));
+    uint32_t* off_h = C.off_h.as<uint32_t>();
+    uint32_t* cd_h = nullptr;
+    float* cdist_h = nullptr;
+    if (n == 0 || nsteps <= 1) std::fill(off_h, off_h + nt + 1, 0u);   // no pairs
     if (n > 0 && nsteps > 1) {
         std::vector<uint32_t> step_of(n);
         for (uint32_t s = 0; s < nsteps; ++s)
             for (uint32_t i = step_off[s]; i < step_off[s + 1]; ++i) step_of[i] = s;
-        DevArena B;
-        float *d_y, *d_x, *d_dist;
-        uint32_t *d_stepof, *d_stepoff, *d_k0, *d_v0, *d_k1, *d_v1, *d_cnt, *d_off, *d_d;
-        char *d_sort_tmp, *d_scan_tmp;
-        LK_TRY(B.alloc(&d_y, n));
-        LK_TRY(B.alloc(&d_x, n));
-        LK_TRY(B.alloc(&d_stepof, n));
-        LK_TRY(B.alloc(&d_stepoff, nsteps + 1));
-        LK_TRY(B.alloc(&d_k0, n));
-        LK_TRY(B.alloc(&d_v0, n));
-        LK_TRY(B.alloc(&d_k1, n));
-        LK_TRY(B.alloc(&d_v1, n));
-        LK_TRY(B.alloc(&d_cnt, nt));
-        LK_TRY(B.alloc(&d_off, nt));
+        LK_TRY(C.y.ensure((size_t)n * 4));
+        LK_TRY(C.x.ensure((size_t)n * 4));
+        LK_TRY(C.stepof.ensure((size_t)n * 4));
+        LK_TRY(C.stepoff.ensure((size_t)(nsteps + 1) * 4));
+        LK_TRY(C.k0.ensure((size_t)n * 4));
+        LK_TRY(C.v0.ensure((size_t)n * 4));
+        LK_TRY(C.k1.ensure((size_t)n * 4));
+        LK_TRY(C.v1.ensure((size_t)n * 4));
+        LK_TRY(C.cnt.ensure(nt * 4));
+        LK_TRY(C.off.ensure(nt * 4));
+        float *d_y = C.y.as<float>(), *d_x = C.x.as<float>();
+        uint32_t *d_stepof = C.stepof.as<uint32_t>(), *d_stepoff = C.stepoff.as<uint32_t>();
+        uint32_t *d_k0 = C.k0.as<uint32_t>(), *d_v0 = C.v0.as<uint32_t>();
+        uint32_t *d_k1 = C.k1.as<uint32_t>(), *d_v1 = C.v1.as<uint32_t>();
+        uint32_t *d_cnt = C.cnt.as<uint32_t>(), *d_off = C.off.as<uint32_t>();
         LK_TRY(hipMemcpyAsync(d_y, ys, (size_t)n * 4, hipMemcpyHostToDevice, st));
         LK_TRY(hipMemcpyAsync(d_x, xs, (size_t)n * 4, hipMemcpyHostToDevice, st));
         LK_TRY(hipMemcpyAsync(d_stepof, step_of.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
@@ -246,8 +269,8 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
         size_t sort_bytes = 0, scan_bytes = 0;
         LK_TRY(rocprim::segmented_radix_sort_pairs(nullptr, sort_bytes, d_k0, d_k1, d_v0, d_v1, n,
                                                    nsteps, d_stepoff, d_stepoff + 1, 0, 32, st));
-        LK_TRY(B.alloc(&d_sort_tmp, sort_bytes));
-        LK_TRY(rocprim::segmented_radix_sort_pairs((void*)d_sort_tmp, sort_bytes, d_k0, d_k1, d_v0,
+        LK_TRY(C.sort_tmp.ensure(sort_bytes));
+        LK_TRY(rocprim::segmented_radix_sort_pairs(C.sort_tmp.p, sort_bytes, d_k0, d_k1, d_v0,
                                                    d_v1, n, nsteps, d_stepoff, d_stepoff + 1, 0,
                                                    32, st));
         const uint32_t blocks = (uint32_t)((nt + 255) / 256);
@@ -257,30 +280,31 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
         LK_TRY(hipGetLastError());
         LK_TRY(rocprim::exclusive_scan(nullptr, scan_bytes, d_cnt, d_off, 0u, nt,
                                        rocprim::plus<uint32_t>(), st));
-        LK_TRY(B.alloc(&d_scan_tmp, scan_bytes));
-        LK_TRY(rocprim::exclusive_scan((void*)d_scan_tmp, scan_bytes, d_cnt, d_off, 0u, nt,
+        LK_TRY(C.scan_tmp.ensure(scan_bytes));
+        LK_TRY(rocprim::exclusive_scan(C.scan_tmp.p, scan_bytes, d_cnt, d_off, 0u, nt,
                                        rocprim::plus<uint32_t>(), st));
-        uint32_t last = 0;
-        LK_TRY(hipMemcpyAsync(off_h.data(), d_off, nt * 4, hipMemcpyDeviceToHost, st));
-        LK_TRY(hipMemcpyAsync(&last, d_cnt + nt - 1, 4, hipMemcpyDeviceToHost, st));
+        LK_TRY(hipMemcpyAsync(off_h, d_off, nt * 4, hipMemcpyDeviceToHost, st));
+        LK_TRY(hipMemcpyAsync(off_h + nt, d_cnt + nt - 1, 4, hipMemcpyDeviceToHost, st));
         LK_TRY(hipStreamSynchronize(st));
-        const uint64_t total = (uint64_t)off_h[nt - 1] + last;
+        const uint64_t total = (uint64_t)off_h[nt - 1] + off_h[nt];
         if (total >= (1ull << 32)) {
             err = "link: more than 2^32 - 1 candidate pairs";
             return hipErrorInvalidValue;
         }
         off_h[nt] = (uint32_t)total;
-        cd_h.resize(total);
-        cdist_h.resize(total);
         if (total) {
-            LK_TRY(B.alloc(&d_d, total));
-            LK_TRY(B.alloc(&d_dist, total));
+            LK_TRY(C.cd.ensure(total * 4));
+            LK_TRY(C.dist.ensure(total * 4));
+            LK_TRY(C.cd_h.ensure(total * 4));
+            LK_TRY(C.dist_h.ensure(total * 4));
+            cd_h = C.cd_h.as<uint32_t>();
+            cdist_h = C.dist_h.as<float>();
             hipLaunchKernelGGL(k_link_pairs, dim3(blocks), dim3(256), 0, st, d_y, d_x, d_stepof,
                                d_stepoff, nsteps, d_k1, d_v1, nm, search_range, n, d_cnt,
-                               (const uint32_t*)d_off, d_d, d_dist);
+                               (const uint32_t*)d_off, C.cd.as<uint32_t>(), C.dist.as<float>());
             LK_TRY(hipGetLastError());
-            LK_TRY(hipMemcpyAsync(cd_h.data(), d_d, total * 4, hipMemcpyDeviceToHost, st));
-            LK_TRY(hipMemcpyAsync(cdist_h.data(), d_dist, total * 4, hipMemcpyDeviceToHost, st));
+            LK_TRY(hipMemcpyAsync(cd_h, C.cd.p, total * 4, hipMemcpyDeviceToHost, st));
+            LK_TRY(hipMemcpyAsync(cdist_h, C.dist.p, total * 4, hipMemcpyDeviceToHost, st));
             LK_TRY(hipStreamSynchronize(st));
         }
         if (stats) stats->pairs = total;
@@ -448,7 +472,8 @@ hipError_t link_steps(hipStream_t st, const float* ys, const float* xs,
     return hipSuccess;
 }
 
-hipError_t link_rows(hipStream_t st, const float* rows, uint64_t n_rows, uint32_t ncols,
+hipError_t link_rows(hipStream_t st, LinkCache* cache, const float* rows, uint64_t n_rows,
+                     uint32_t ncols,
                      float search_range, uint32_t memory, uint64_t* ids, LinkStats* stats,
                      std::string& err) {
     if (n_rows >= (1ull << 32)) {
@@ -472,7 +497,8 @@ hipError_t link_rows(hipStream_t st, const float* rows, uint64_t n_rows, uint32_
         xs[i] = rows[i * ncols + 3];
     }
     if (n_rows > 0) step_off.push_back((uint32_t)n_rows);
-    return link_steps(st, ys.data(), xs.data(), step_off, search_range, memory, ids, stats, err);
+    return link_steps(st, cache, ys.data(), xs.data(), step_off, search_range, memory, ids, stats,
+                      err);
 }
 
 #undef LK_TRY
