@@ -12,7 +12,7 @@ import pytest
 
 from oracle.linker import Linker, link_rows
 from tracking_optimizations_amd import _lib, synth
-from tracking_optimizations_amd.gpu_tracking import batch_rust, link_rust
+from tracking_optimizations_amd.gpu_tracking import batch_rust, link_rust, make_params
 
 pytestmark = pytest.mark.gpu
 
@@ -150,3 +150,28 @@ def test_full_size_separated_tracks(ctx):
     assert np.array_equal(relabel(ids), relabel(exp))
     st = ctx.link_stats()
     assert st["steps"] == n_frames and st["max_subnet"] <= 1
+
+
+def test_chunked_linking_matches_oracle(ctx):
+    """600 frames of 512^2 span three locate chunks (256 MiB of input each);
+    each chunk is linked on a worker thread while the next one is located.
+    Device-resident and host frames must both give the per-frame oracle's
+    ids over the whole movie."""
+    import ctypes
+    spec = synth.link_1024(seed=31, n_frames=600, n_spots=150, size=512)
+    movie = synth.generate(ctx, spec)
+    video = movie.to_host(np.uint16, (600, 512, 512))
+    p = make_params(9, separation=9, minmass=300.0, search_range=4.0, memory=2)
+    out = ctypes.c_void_p()
+    _lib.check(_lib.load().gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), 600, 512, 512,
+                                         ctypes.byref(p), _lib.GT_FRAMES_ON_DEVICE,
+                                         ctypes.byref(out)))
+    dev_rows = _lib.Result(out).rows()
+    movie.free()
+    host_rows, cols = batch_rust(video, 9, separation=9, minmass=300.0, search_range=4.0,
+                                 memory=2)
+    np.testing.assert_array_equal(dev_rows, host_rows)
+    plain_rows, _ = batch_rust(video, 9, separation=9, minmass=300.0)
+    assert np.array_equal(host_rows[:, :4], plain_rows[:, :4])
+    ref = _oracle_per_frame(plain_rows, 600, 4.0, 2)
+    assert np.array_equal(host_rows[:, 4], ref.astype(np.float32))

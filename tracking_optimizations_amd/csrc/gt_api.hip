@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <future>
 #include <map>
@@ -542,6 +543,100 @@ gt_status link_result(gt_context* ctx, const std::vector<FrameIn>& frames, const
     res->ncols = nc + 1;
     return GT_OK;
 }
+
+// Linking beside locate: chunks of located rows are linked in order on a
+// worker thread (its own stream and buffers) while the next chunk is being
+// located; Linker::advance runs once per frame, as in link_result.
+struct LinkPipeline {
+    gt_context* ctx;
+    const gt_params* p;
+    LinkState* state;
+    std::future<std::pair<gt_status, std::string>> tail;
+    std::deque<std::vector<uint64_t>> ids;
+    LinkStats total{};
+    LinkPipeline(gt_context* c, const gt_params* prm, uint64_t n_frames) : ctx(c), p(prm) {
+        const uint64_t mem = prm->has_memory ? prm->memory : 0;
+        state = link_state_create((uint32_t)std::min<uint64_t>(mem, std::max<uint64_t>(n_frames, 1)));
+        if (!ctx->link_cache) ctx->link_cache = link_cache_create();
+    }
+    ~LinkPipeline() {
+        if (tail.valid()) tail.wait();
+        link_state_destroy(state);
+    }
+    // rows: the chunk's rows (nc columns, frame order), frames: its frames
+    gt_status submit(const std::vector<FrameIn>& frames, const float* rows, uint64_t nrows,
+                     uint32_t nc) {
+        std::vector<uint32_t> so(frames.size() + 1, 0);
+        std::vector<float> ys(nrows), xs(nrows);
+        uint64_t r = 0;
+        for (size_t f = 0; f < frames.size(); ++f) {
+            const float fv = (float)frames[f].fi;
+            for (; r < nrows && rows[r * nc] == fv; ++r) {
+                ys[r] = rows[r * nc + 2];
+                xs[r] = rows[r * nc + 3];
+            }
+            so[f + 1] = (uint32_t)r;
+        }
+        if (r != nrows) return fail(GT_E_INTERNAL, "link: rows are not in frame order");
+        ids.emplace_back(nrows);
+        uint64_t* out = ids.back().data();
+        auto prev = std::move(tail);
+        tail = std::async(std::launch::async,
+                          [this, prev = std::move(prev), ys = std::move(ys), xs = std::move(xs),
+                           so = std::move(so), out]() mutable -> std::pair<gt_status, std::string> {
+                              if (prev.valid()) {
+                                  auto pr = prev.get();
+                                  if (pr.first != GT_OK) return pr;
+                              }
+                              if (hipSetDevice(ctx->device) != hipSuccess)
+                                  return {GT_E_HIP, "link: hipSetDevice failed"};
+                              hipStream_t st = nullptr;
+                              std::string err;
+                              hipError_t e = link_cache_stream(ctx->link_cache, &st);
+                              LinkStats ls;
+                              if (e == hipSuccess)
+                                  e = link_chunk(st, ctx->link_cache, state, ys.data(), xs.data(), so,
+                                                 p->search_range, out, &ls, err);
+                              if (e != hipSuccess)
+                                  return {e == hipErrorOutOfMemory ? GT_E_OOM
+                                          : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP,
+                                          err.empty() ? hipGetErrorString(e) : err};
+                              total.steps += ls.steps;
+                              total.pairs += ls.pairs;
+                              total.subnets += ls.subnets;
+                              total.max_subnet = std::max(total.max_subnet, ls.max_subnet);
+                              total.ids = ls.ids;
+                              total.search_ms += ls.search_ms;
+                              total.solve_ms += ls.solve_ms;
+                              return {GT_OK, {}};
+                          });
+        return GT_OK;
+    }
+    // waits for the last chunk, then adds the particle column (after x) to res
+    gt_status finish(gt_result* res) {
+        if (tail.valid()) {
+            auto r = tail.get();
+            if (r.first != GT_OK) return fail(r.first, "%s", r.second.c_str());
+        }
+        ctx->last_link = total;
+        const uint32_t nc = res->ncols;
+        std::vector<float> out(res->rows * (nc + 1));
+        uint64_t i = 0;
+        for (const auto& v : ids)
+            for (uint64_t id : v) {
+                const float* src = res->data.data() + i * nc;
+                float* dst = out.data() + i * (nc + 1);
+                for (uint32_t c = 0; c < 4; ++c) dst[c] = src[c];
+                dst[4] = (float)id;
+                for (uint32_t c = 4; c < nc; ++c) dst[c + 1] = src[c];
+                ++i;
+            }
+        if (i != res->rows) return fail(GT_E_INTERNAL, "link: row count mismatch");
+        res->data.swap(out);
+        res->ncols = nc + 1;
+        return GT_OK;
+    }
+};
 
 gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
                      const gt_params* p, uint32_t flags, gt_result** out,
@@ -1315,14 +1410,17 @@ gt_status gt_context_set_batch(gt_context* ctx, uint32_t frames_per_launch) {
     return GT_OK;
 }
 
-// Host-resident frames, located chunk by chunk (about 256 MiB each) through
-// the context's two pinned buffers: while chunk c is uploaded and located,
-// up to 8 threads fill chunk c + 1 (fill(frame, dst) copies or decodes one
-// frame).  With search_range the rows are linked once over all frames.
+// Frames located chunk by chunk (about 256 MiB of input each).  Host frames
+// (fill given) go through the context's two pinned buffers: while chunk c is
+// uploaded and located, up to 8 threads fill chunk c + 1 (fill(frame, dst)
+// copies or decodes one frame).  Device frames (fill empty) are used in
+// place.  With search_range each located chunk is linked on a worker thread
+// while the next one is located.
 static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& frames, int in_type,
                                const gt_params* p, uint32_t flags,
                                const std::function<gt_status(size_t, void*)>& fill,
                                gt_result** out) {
+    const bool host = (bool)fill;
     const auto t_start = std::chrono::steady_clock::now();
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t es = elem_size(in_type);
@@ -1342,7 +1440,7 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
     max_chunk = std::max(max_chunk, cur);
     starts.push_back(frames.size());
     const size_t nchunks = frames.empty() ? 0 : starts.size() - 1;
-    if (nchunks && ctx->pinned_bytes < max_chunk) {
+    if (host && nchunks && ctx->pinned_bytes < max_chunk) {
         for (auto& b : ctx->pinned) {
             if (b) (void)hipHostFree(b);
             b = nullptr;
@@ -1387,21 +1485,25 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         res->ncols = nc;
     }
     gt_timing total{};
-    if (nchunks) {
+    std::unique_ptr<LinkPipeline> lp;
+    if (p->has_search_range) lp.reset(new LinkPipeline(ctx, p, frames.size()));
+    if (host && nchunks) {
         const Status r0 = fill_chunk(0);
         if (r0.first != GT_OK) return fail(r0.first, "%s", r0.second.c_str());
     }
     for (size_t c = 0; c < nchunks; ++c) {
         std::future<Status> next;
-        if (c + 1 < nchunks) next = std::async(std::launch::async, fill_chunk, c + 1);
+        if (host && c + 1 < nchunks) next = std::async(std::launch::async, fill_chunk, c + 1);
         std::vector<FrameIn> fr(frames.begin() + starts[c], frames.begin() + starts[c + 1]);
-        for (size_t i = 0; i < fr.size(); ++i) fr[i].ptr = buf[c & 1] + offs[starts[c] + i];
+        if (host)
+            for (size_t i = 0; i < fr.size(); ++i) fr[i].ptr = buf[c & 1] + offs[starts[c] + i];
         gt_result* part = nullptr;
         const gt_status st = run_locate(ctx, fr, in_type, &q, flags, &part);
         const Status rs = next.valid() ? next.get() : Status{GT_OK, {}};
         if (st != GT_OK) return st;
         std::unique_ptr<gt_result> partp(part);
         if (rs.first != GT_OK) return fail(rs.first, "%s", rs.second.c_str());
+        if (lp) GT_TRY(lp->submit(fr, part->data.data(), part->rows, part->ncols));
         res->data.insert(res->data.end(), part->data.begin(), part->data.end());
         res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
         res->rows += part->rows;
@@ -1412,7 +1514,7 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         total.rows += ctx->last.rows;
         total.launches += ctx->last.launches;
     }
-    if (p->has_search_range) GT_TRY(link_result(ctx, frames, p, res.get()));
+    if (lp) GT_TRY(lp->finish(res.get()));
     {
         const char* names[16];
         const char* dtypes[16];
@@ -1429,9 +1531,14 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
 }
 
 // Host frames go through the pinned staging unless the call asks for the
-// per-frame path, device-resident rows or an adjoint.
+// per-frame path or device-resident rows; device frames are chunked only to
+// link beside locate.
 static bool staged_path(uint32_t flags) {
     return !(flags & (GT_FRAMES_ON_DEVICE | GT_PLAIN | GT_RESULT_ON_DEVICE));
+}
+static bool chunked_link_path(uint32_t flags, const gt_params* p) {
+    return p && p->has_search_range && (flags & GT_FRAMES_ON_DEVICE) &&
+           !(flags & (GT_PLAIN | GT_RESULT_ON_DEVICE));
 }
 
 static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n, uint32_t h,
@@ -1451,6 +1558,10 @@ static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n,
             std::memcpy(dst, v[fi].ptr, fb);
             return GT_OK;
         }, out);
+    }
+    if (chunked_link_path(flags, p) && n > 0 && ctx && out) {
+        GT_TRY(check_params(p, h, w));
+        return locate_staged(ctx, v, in_type, p, flags, {}, out);
     }
     return run_locate(ctx, v, in_type, p, flags, out);
 }
@@ -1482,7 +1593,7 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
     v.reserve(n_frames);
     for (uint64_t i = 0; i < n_frames; ++i)
         v.push_back({frames[i].data, frames[i].h, frames[i].w, frames[i].frame_index});
-    if (staged_path(flags) && n_frames > 0 && ctx && out) {
+    if ((staged_path(flags) || chunked_link_path(flags, p)) && n_frames > 0 && ctx && out) {
         for (const auto& f : v) {
             GT_TRY(check_params(p, f.h, f.w));
             if (!f.ptr) return fail(GT_E_INVALID, "frame pointer is NULL");
@@ -1492,6 +1603,7 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
                 if ((float)v[f].fi == (float)v[f - 1].fi)
                     return fail(GT_E_INVALID, "search_range needs consecutive frames with distinct indices");
         const size_t es = elem_size(in_type);
+        if (!staged_path(flags)) return locate_staged(ctx, v, in_type, p, flags, {}, out);
         return locate_staged(ctx, v, in_type, p, flags, [&](size_t fi, void* dst) {
             std::memcpy(dst, v[fi].ptr, (size_t)v[fi].h * v[fi].w * es);
             return GT_OK;

@@ -185,6 +185,29 @@ struct LinkStats {
 struct LinkCache;
 LinkCache* link_cache_create();
 void link_cache_destroy(LinkCache* c);
+// A stream of the cache's own (default priority), created on first use with
+// the calling thread's device current.
+hipError_t link_cache_stream(LinkCache* c, hipStream_t* out);
+// Linker state between chunks of steps: the last step's particles and the
+// memory entries (oldest step first), the next id, the steps seen so far.
+struct LinkState {
+    struct Carry {
+        float y, x;
+        uint64_t id;
+    };
+    uint32_t memory = 0;
+    uint64_t counter = 0, steps_done = 0;
+    std::vector<Carry> prev;
+    std::vector<std::vector<Carry>> ring;   // `memory` steps, oldest first
+    std::vector<uint32_t> stamp;            // per id, solver scratch
+};
+LinkState* link_state_create(uint32_t memory);
+void link_state_destroy(LinkState* s);
+// Links the steps of one chunk (soff: chunk-local step offsets) after the
+// steps the state has seen; ids of the chunk's rows go to ids.
+hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const float* ys,
+                      const float* xs, const std::vector<uint32_t>& soff, float search_range,
+                      uint64_t* ids, LinkStats* stats, std::string& err);
 hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const float* xs,
                       const std::vector<uint32_t>& step_off, float search_range,
                       uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err);

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -199,10 +200,22 @@ struct Buf {
 struct LinkCache {
     Buf<false> y, x, stepof, stepoff, k0, v0, k1, v1, cnt, off, cd, dist, sort_tmp, scan_tmp;
     Buf<true> off_h, cd_h, dist_h;
+    hipStream_t stream = nullptr;   // for linking beside the locate streams
+    ~LinkCache() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 LinkCache* link_cache_create() { return new LinkCache(); }
 void link_cache_destroy(LinkCache* c) { delete c; }
+hipError_t link_cache_stream(LinkCache* c, hipStream_t* out) {
+    if (!c->stream) {
+        const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+    }
+    *out = c->stream;
+    return hipSuccess;
+}
 
 #define LK_TRY(x)                                                 \
     do {                                                          \
@@ -213,15 +226,46 @@ void link_cache_destroy(LinkCache* c) { delete c; }
         }                                                         \
     } while (0)
 
-hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const float* xs,
-                      const std::vector<uint32_t>& step_off, float search_range,
-                      uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err) {
-    const uint32_t nsteps = (uint32_t)step_off.size() - 1;
-    const uint32_t n = step_off.back();
+LinkState* link_state_create(uint32_t memory) {
+    LinkState* s = new LinkState();
+    s->memory = memory;
+    s->ring.resize(memory);
+    return s;
+}
+void link_state_destroy(LinkState* s) { delete s; }
+
+hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const float* ys_c,
+                      const float* xs_c, const std::vector<uint32_t>& soff_c, float search_range,
+                      uint64_t* ids, LinkStats* stats, std::string& err) {
     if (stats) *stats = LinkStats{};
-    // memory beyond the number of steps behaves as memory == steps: entries
-    // never expire within the call
-    memory = std::min(memory, nsteps);
+    const uint32_t memory = state->memory;
+    const uint32_t nsc = (uint32_t)soff_c.size() - 1, nc = soff_c.back();
+    // Table of positions: steps 0..memory-1 hold the memory entries carried
+    // in (oldest first), step memory the previous step's particles, then
+    // the chunk's steps; a carried particle's step keeps its age relative to
+    // the chunk, so (row, age) candidate lists mean the same as in one call.
+    std::vector<uint32_t> step_off(1, 0);
+    std::vector<float> ysv, xsv;
+    std::vector<uint64_t> carry_id;
+    auto carry = [&](const std::vector<LinkState::Carry>& v) {
+        for (const auto& e : v) {
+            ysv.push_back(e.y);
+            xsv.push_back(e.x);
+            carry_id.push_back(e.id);
+        }
+        step_off.push_back((uint32_t)ysv.size());
+    };
+    for (uint32_t k = 0; k < memory; ++k) carry(state->ring[k]);
+    carry(state->prev);
+    const uint32_t ncar = (uint32_t)ysv.size();
+    ysv.insert(ysv.end(), ys_c, ys_c + nc);
+    xsv.insert(xsv.end(), xs_c, xs_c + nc);
+    for (uint32_t k = 1; k <= nsc; ++k) step_off.push_back(ncar + soff_c[k]);
+    const float* ys = ysv.data();
+    const float* xs = xsv.data();
+    const uint32_t nsteps = (uint32_t)step_off.size() - 1;
+    const uint32_t first = memory + 1;   // first step of the chunk
+    const uint32_t n = step_off.back();
     const uint32_t nm = memory + 1;
     const size_t nt = (size_t)n * nm;
     if (nt >= (1ull << 32)) {
@@ -229,7 +273,7 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
         return hipErrorInvalidValue;
     }
 
-    if (stats) stats->steps = nsteps;
+    if (stats) stats->steps = nsc;
     const auto t0 = std::chrono::steady_clock::now();
 
     // ---- candidates on the device --------------------------------------------
@@ -315,12 +359,17 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
     // ---- Linker::advance over the steps --------------------------------------
     // Ids are dense (< n); a per-id stamp answers "is this id among the
     // sources" (2t + 1) and "was it handed to a destination" (2t + 2).
-    std::vector<uint32_t> stamp((size_t)n + 1, 0);
+    uint64_t counter = state->counter;
+    std::vector<uint32_t>& stamp = state->stamp;
+    if (stamp.size() < counter + nc + 1) stamp.resize(counter + nc + 1, 0);
     std::vector<Src> prev, next;
-    uint32_t prev_n = 0;
     std::vector<std::vector<Src>> ring(memory);   // memory steps, ring[head] oldest
     uint32_t head = 0;
-    uint64_t counter = 0;
+    for (uint32_t k = 0; k < memory; ++k)
+        for (uint32_t i = step_off[k]; i < step_off[k + 1]; ++i) ring[k].push_back({i, k, carry_id[i]});
+    for (uint32_t i = step_off[memory]; i < step_off[memory + 1]; ++i)
+        prev.push_back({i, memory, carry_id[i]});
+    uint32_t prev_n = (uint32_t)prev.size();
     std::vector<uint32_t> d2s_off, d2s_fill, d2s_src;
     std::vector<float> d2s_dist;
     std::vector<uint8_t> vis_s, vis_d;
@@ -340,9 +389,11 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
         b = off_h[key];
         e = off_h[key + 1];
     };
-    for (uint32_t t = 0; t < nsteps; ++t) {
+    for (uint32_t t = first; t < nsteps; ++t) {
         const uint32_t a = step_off[t], nd = step_off[t + 1] - step_off[t];
-        const uint32_t st_src = 2 * t + 1, st_out = 2 * t + 2;
+        // stamps are unique over every step the state has seen
+        const uint64_t gs = state->steps_done + (t - first);
+        const uint32_t st_src = (uint32_t)(2 * gs + 1), st_out = (uint32_t)(2 * gs + 2);
         // memory entries, oldest first, ids not already among the sources
         if (memory > 0) {
             for (const Src& s : prev) stamp[s.id] = st_src;
@@ -445,7 +496,7 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
         next.resize(nd);
         for (uint32_t i = 0; i < nd; ++i) {
             const uint64_t id = out[i] >= 0 ? (uint64_t)out[i] : counter++;
-            ids[a + i] = id;
+            ids[a + i - ncar] = id;
             stamp[id] = st_out;
             next[i] = {a + i, t, id};
         }
@@ -461,6 +512,15 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
         std::swap(prev, next);
         prev_n = nd;
     }
+    // the state carried to the next chunk
+    auto to_carry = [&](const std::vector<Src>& v, std::vector<LinkState::Carry>& out_v) {
+        out_v.clear();
+        for (const Src& e : v) out_v.push_back({ys[e.p], xs[e.p], e.id});
+    };
+    to_carry(prev, state->prev);
+    for (uint32_t k = 0; k < memory; ++k) to_carry(ring[(head + k) % memory], state->ring[k]);
+    state->counter = counter;
+    state->steps_done += nsc;
     if (stats) {
         using ms = std::chrono::duration<double, std::milli>;
         stats->search_ms = ms(t1 - t0).count();
@@ -470,6 +530,16 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
         stats->ids = counter;
     }
     return hipSuccess;
+}
+
+hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const float* xs,
+                      const std::vector<uint32_t>& step_off, float search_range,
+                      uint32_t memory, uint64_t* ids, LinkStats* stats, std::string& err) {
+    // memory beyond the number of steps behaves as memory == steps: entries
+    // never expire within the call
+    const uint32_t nsteps = (uint32_t)step_off.size() - 1;
+    std::unique_ptr<LinkState> state(link_state_create(std::min(memory, nsteps)));
+    return link_chunk(st, cache, state.get(), ys, xs, step_off, search_range, ids, stats, err);
 }
 
 hipError_t link_rows(hipStream_t st, LinkCache* cache, const float* rows, uint64_t n_rows,
