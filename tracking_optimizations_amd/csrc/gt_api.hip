@@ -1504,6 +1504,11 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         std::unique_ptr<gt_result> partp(part);
         if (rs.first != GT_OK) return fail(rs.first, "%s", rs.second.c_str());
         if (lp) GT_TRY(lp->submit(fr, part->data.data(), part->rows, part->ncols));
+        if (c == 0 && nchunks > 1) {   // room for all chunks at the first one's density
+            const size_t est = (size_t)((double)part->rows * frames.size() / fr.size() * 1.1);
+            res->data.reserve(est * part->ncols);
+            res->cand.reserve(est);
+        }
         res->data.insert(res->data.end(), part->data.begin(), part->data.end());
         res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
         res->rows += part->rows;

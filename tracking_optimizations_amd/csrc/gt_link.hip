@@ -376,6 +376,7 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
     std::vector<std::pair<uint32_t, uint8_t>> stack;
     std::vector<uint32_t> path_d, sub_dests, sub_off;
     std::vector<int64_t> out;
+    std::vector<std::pair<uint32_t, uint32_t>> lists;
     Assign A;
     A.null_cost = search_range * search_range;
     uint64_t n_subnets = 0, max_subnet = 0;
@@ -394,24 +395,28 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
         // stamps are unique over every step the state has seen
         const uint64_t gs = state->steps_done + (t - first);
         const uint32_t st_src = (uint32_t)(2 * gs + 1), st_out = (uint32_t)(2 * gs + 2);
-        // memory entries, oldest first, ids not already among the sources
+        // memory entries, oldest first, ids not already among the sources:
+        // the previous step's ids carry that step's output stamp (2 gs),
+        // memory entries taken in this step get st_src
         if (memory > 0) {
-            for (const Src& s : prev) stamp[s.id] = st_src;
+            const uint32_t st_prev = (uint32_t)(2 * gs);
             for (uint32_t k = 0; k < memory; ++k)
                 for (const Src& e : ring[(head + k) % memory]) {
-                    if (stamp[e.id] == st_src) continue;
+                    if (stamp[e.id] == st_src || (gs > 0 && stamp[e.id] == st_prev)) continue;
                     stamp[e.id] = st_src;
                     prev.push_back(e);
                 }
         }
         const uint32_t ns = (uint32_t)prev.size();
+        // each source's candidate list [b, e) for this step
+        lists.resize(ns);
+        for (uint32_t si = 0; si < ns; ++si) list_of(prev[si], t, lists[si].first, lists[si].second);
         // every source's list in ascending destination order (each list
         // belongs to one step only, so it is sorted in place), and the
         // destination -> source lists in ascending source order
         d2s_off.assign((size_t)nd + 1, 0);
         for (uint32_t si = 0; si < ns; ++si) {
-            uint32_t b, e;
-            list_of(prev[si], t, b, e);
+            const uint32_t b = lists[si].first, e = lists[si].second;
             for (uint32_t q = b + 1; q < e; ++q) {   // insertion sort, lists are short
                 const uint32_t d = cd_h[q];
                 const float v = cdist_h[q];
@@ -430,8 +435,7 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
         d2s_src.resize(d2s_off[nd]);
         d2s_dist.resize(d2s_off[nd]);
         for (uint32_t si = 0; si < ns; ++si) {
-            uint32_t b, e;
-            list_of(prev[si], t, b, e);
+            const uint32_t b = lists[si].first, e = lists[si].second;
             for (uint32_t q = b; q < e; ++q) {
                 const uint32_t w = d2s_fill[cd_h[q]]++;
                 d2s_src[w] = si;
@@ -447,6 +451,23 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
         sub_off.assign(1, 0);
         for (uint32_t i = 0; i < nd; ++i) {
             if (vis_d[i]) continue;
+            // the common subnets resolved without the stack: a destination
+            // without candidates, or one whose only source has only it
+            // (what the search would find: path [i] and at most one source)
+            const uint32_t nsi = d2s_off[i + 1] - d2s_off[i];
+            if (nsi == 0) {
+                vis_d[i] = 1;
+                continue;
+            }
+            if (nsi == 1) {
+                const uint32_t si = d2s_src[d2s_off[i]];
+                if (lists[si].second - lists[si].first == 1) {
+                    vis_d[i] = 1;
+                    vis_s[si] = 1;
+                    out[i] = (int64_t)prev[si].id;
+                    continue;
+                }
+            }
             path_d.clear();
             uint32_t n_src = 0, first_src = 0;
             stack.assign(1, {i, 1});
@@ -457,8 +478,7 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
                     if (vis_s[idx]) continue;
                     vis_s[idx] = 1;
                     if (n_src++ == 0) first_src = idx;
-                    uint32_t b, e;
-                    list_of(prev[idx], t, b, e);
+                    const uint32_t b = lists[idx].first, e = lists[idx].second;
                     for (uint32_t q = e; q-- > b;) stack.push_back({cd_h[q], 1});
                 } else {
                     if (vis_d[idx]) continue;
