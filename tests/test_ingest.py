@@ -118,3 +118,16 @@ def test_ets_errors(tmp_path):
     f.write_bytes(bytes(data))
     with pytest.raises(GtInvalid):
         read_movie(str(f))
+
+
+def test_tiff_ifd_loop_is_rejected(tmp_path):
+    import struct
+    f = tmp_path / "loop.tif"
+    write_tiff(f, _pages(np.uint16, n=1))
+    data = bytearray(f.read_bytes())
+    (ifd,) = struct.unpack_from("<I", data, 4)
+    (n,) = struct.unpack_from("<H", data, ifd)
+    struct.pack_into("<I", data, ifd + 2 + 12 * n, ifd)    # next IFD = itself
+    f.write_bytes(bytes(data))
+    with pytest.raises(GtInvalid):
+        read_movie(str(f))

@@ -14,6 +14,8 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -50,6 +52,21 @@ gt_status fail(gt_status s, const char* fmt, ...) {
         gt_status s_ = (expr);        \
         if (s_ != GT_OK) return s_;   \
     } while (0)
+
+// C entry points turn C++ exceptions (host allocation failures, thread
+// creation) into status codes instead of letting them cross the C ABI.
+template <class F>
+gt_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(GT_E_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(GT_E_INTERNAL, "%s", e.what());
+    } catch (...) {
+        return fail(GT_E_INTERNAL, "unknown exception");
+    }
+}
 
 struct DevBuf {
     void* p = nullptr;
@@ -1315,6 +1332,7 @@ gt_status gt_context_set_graphs(gt_context* ctx, int enable) {
 
 gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, uint32_t w,
                               uint32_t n_frames, gt_tuning* out) {
+    return guarded([&]() -> gt_status {
     if (!ctx || !p) return fail(GT_E_INVALID, "NULL argument");
     GT_TRY(check_params(p, h, w));
     if (n_frames == 0) n_frames = 1;
@@ -1396,6 +1414,7 @@ gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, u
         out->device_ms_per_frame = best / n_frames;
     }
     return GT_OK;
+    });
 }
 
 gt_status gt_context_set_fused_walk(gt_context* ctx, int enable) {
@@ -1573,21 +1592,28 @@ static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n,
 
 gt_status gt_locate_u16(gt_context* ctx, const uint16_t* frames, uint64_t n_frames, uint32_t h,
                         uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return guarded([&]() -> gt_status {
     return locate_uniform(ctx, frames, n_frames, h, w, IN_U16, p, flags, out);
+    });
 }
 
 gt_status gt_locate_u8(gt_context* ctx, const uint8_t* frames, uint64_t n_frames, uint32_t h,
                        uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return guarded([&]() -> gt_status {
     return locate_uniform(ctx, frames, n_frames, h, w, IN_U8, p, flags, out);
+    });
 }
 
 gt_status gt_locate_f32(gt_context* ctx, const float* frames, uint64_t n_frames, uint32_t h,
                         uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
+    return guarded([&]() -> gt_status {
     return locate_uniform(ctx, frames, n_frames, h, w, IN_F32, p, flags, out);
+    });
 }
 
 gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t n_frames,
                           int dtype, const gt_params* p, uint32_t flags, gt_result** out) {
+    return guarded([&]() -> gt_status {
     if (!frames && n_frames) return fail(GT_E_INVALID, "frames is NULL");
     int in_type;
     if (dtype == 0) in_type = IN_U16;
@@ -1615,10 +1641,12 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames, uint64_t
         }, out);
     }
     return run_locate(ctx, v, in_type, p, flags, out);
+    });
 }
 
 gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel, const gt_params* p,
                          uint32_t flags, gt_result** out) {
+    return guarded([&]() -> gt_status {
     if (!out) return fail(GT_E_INVALID, "out is NULL");
     *out = nullptr;
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
@@ -1641,11 +1669,13 @@ gt_status gt_locate_file(gt_context* ctx, const char* path, uint32_t channel, co
     for (uint64_t i = 0; i < n; ++i) all.push_back({nullptr, h, w, (int64_t)i});
     return locate_staged(ctx, all, dtype == 0 ? IN_U16 : IN_U8, p, flags,
                          [fp](size_t fi, void* dst) { return gt_file_read(fp, fi, 1, dst); }, out);
+    });
 }
 
 gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
                             uint32_t h, uint32_t w, const gt_params* p, uint32_t flags,
                             gt_adjoint** out) {
+    return guarded([&]() -> gt_status {
     if (!out) return fail(GT_E_INVALID, "out is NULL");
     *out = nullptr;
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
@@ -1706,6 +1736,7 @@ gt_status gt_locate_adjoint(gt_context* ctx, const void* frames, int dtype, uint
     }
     *out = a;
     return GT_OK;
+    });
 }
 
 uint64_t gt_adjoint_rows(const gt_adjoint* a) { return a ? a->fwd->rows : 0; }
@@ -1735,6 +1766,7 @@ gt_status gt_adjoint_jacobian(const gt_adjoint* a, float* out, uint64_t capacity
 
 gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad_rows,
                          float* grad_frames, uint32_t flags) {
+    return guarded([&]() -> gt_status {
     if (!ctx || !a) return fail(GT_E_INVALID, "NULL argument");
     if (!a->jacobian) return fail(GT_E_INVALID, "the adjoint was computed without GT_ADJ_JACOBIAN");
     if (!grad_frames && a->nframes) return fail(GT_E_INVALID, "grad_frames is NULL");
@@ -1764,6 +1796,7 @@ gt_status gt_adjoint_vjp(gt_context* ctx, const gt_adjoint* a, const float* grad
         e = hipMemcpy(grad_frames, gout, gbytes, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return fail(GT_E_HIP, "gt_adjoint_vjp: %s", hipGetErrorString(e));
     return GT_OK;
+    });
 }
 
 void gt_adjoint_free(gt_adjoint* a) { delete a; }
@@ -1794,6 +1827,7 @@ void gt_result_free(gt_result* r) { delete r; }
 gt_status gt_synth_u16(gt_context* ctx, uint16_t* d_frames, uint64_t n_frames, uint32_t h,
                        uint32_t w, const float* spots, const uint64_t* spot_off, const float* bg,
                        uint64_t seed) {
+    return guarded([&]() -> gt_status {
     if (!ctx || !d_frames || !spot_off || !bg) return fail(GT_E_INVALID, "NULL argument");
     if (h == 0 || w == 0) return fail(GT_E_INVALID, "empty frame");
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1850,6 +1884,7 @@ gt_status gt_synth_u16(gt_context* ctx, uint16_t* d_frames, uint64_t n_frames, u
     d_off.release();
     d_bg.release();
     return GT_OK;
+    });
 }
 
 gt_status gt_device_alloc(gt_context* ctx, uint64_t bytes, void** out) {
@@ -1879,6 +1914,7 @@ gt_status gt_memcpy_d2h(gt_context* ctx, void* dst, const void* src, uint64_t by
 
 gt_status gt_link(gt_context* ctx, const float* rows, uint64_t n_rows, uint32_t ncols,
                   float search_range, uint64_t memory, uint64_t* ids) {
+    return guarded([&]() -> gt_status {
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
     if (n_rows && (!rows || !ids)) return fail(GT_E_INVALID, "NULL argument");
     if (ncols < 4) return fail(GT_E_INVALID, "rows need at least 4 columns (frame, mass, y, x)");
@@ -1892,6 +1928,7 @@ gt_status gt_link(gt_context* ctx, const float* rows, uint64_t n_rows, uint32_t 
         return fail(e == hipErrorOutOfMemory ? GT_E_OOM
                     : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP, "%s", err.c_str());
     return GT_OK;
+    });
 }
 
 gt_status gt_context_link_stats(const gt_context* ctx, gt_link_stats* out) {

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -124,9 +125,11 @@ gt_status open_tiff(gt_file& f) {
     const bool big = magic == 43;
     if (magic != 42 && !big) return set_last_error(GT_E_INVALID, "not a TIFF file");
     if (!R.u(big ? 8 : 4, big ? 8 : 4, ifd)) return set_last_error(GT_E_INVALID, "TIFF: " + R.err);
-    uint64_t guard = 0;
+    std::vector<uint64_t> seen;   // IFD offsets visited (a chain must not loop)
     while (ifd) {
-        if (++guard > (1u << 24)) return set_last_error(GT_E_INVALID, "TIFF: IFD chain does not end");
+        if (std::find(seen.begin(), seen.end(), ifd) != seen.end() || seen.size() > (1u << 24))
+            return set_last_error(GT_E_INVALID, "TIFF: IFD chain does not end");
+        seen.push_back(ifd);
         uint64_t n;
         if (!R.u(ifd, big ? 8 : 2, n)) return set_last_error(GT_E_INVALID, "TIFF: " + R.err);
         const uint64_t esz = big ? 20 : 12;
@@ -404,9 +407,22 @@ gt_status open_ets(gt_file& f, uint32_t channel) {
 
 }  // namespace
 
+// C entry points turn C++ exceptions into status codes.
+template <class F>
+static gt_status guarded_io(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return set_last_error(GT_E_OOM, "host allocation failed");
+    } catch (...) {
+        return set_last_error(GT_E_INTERNAL, "exception in the movie reader");
+    }
+}
+
 extern "C" {
 
 gt_status gt_file_open(const char* path, uint32_t channel, gt_file** out) {
+    return guarded_io([&]() -> gt_status {
     if (!path || !out) return set_last_error(GT_E_INVALID, "NULL argument");
     *out = nullptr;
     std::string p(path);
@@ -433,6 +449,7 @@ gt_status gt_file_open(const char* path, uint32_t channel, gt_file** out) {
     }
     *out = f;
     return GT_OK;
+    });
 }
 
 gt_status gt_file_info(const gt_file* f, int* kind, uint64_t* n_frames, uint32_t* h, uint32_t* w,
@@ -454,6 +471,7 @@ gt_status gt_file_channels(const gt_file* f, uint32_t* channels, uint32_t cap, u
 }
 
 gt_status gt_file_read(const gt_file* f, uint64_t first, uint64_t count, void* dst) {
+    return guarded_io([&]() -> gt_status {
     if (!f || (count && !dst)) return set_last_error(GT_E_INVALID, "NULL argument");
     const uint64_t n = f->kind == 0 ? f->pages.size() : f->ets_off.size();
     if (first > n || count > n - first) return set_last_error(GT_E_INVALID, "frame range out of bounds");
@@ -470,6 +488,7 @@ gt_status gt_file_read(const gt_file* f, uint64_t first, uint64_t count, void* d
         }
     }
     return GT_OK;
+    });
 }
 
 void gt_file_close(gt_file* f) { delete f; }
