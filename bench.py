@@ -96,7 +96,7 @@ def main() -> int:
                          "link: locate + link, 2000 x 1024^2, 3000 blinking particles; "
                          "file: batch_file on a 256 x 2048^2 uint16 TIFF")
     ap.add_argument("--link-cpu-frames", type=int, default=150,
-                    help="frames of the link workload timed with the CPU oracle linker")
+                    help="frames of the link workload also linked by the Python oracle")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -443,8 +443,9 @@ def bench_link(args) -> int:
     median mass of its brightest rows, as many rows as spots drawn),
     search_range 5, memory 3.  A step is one locate call with linking (rows
     with the particle column back on the host).  Beside it: locate alone,
-    gt_link alone on the same row table, and the CPU oracle linker on the
-    first --link-cpu-frames frames."""
+    gt_link alone on the same row table, the C oracle linker on every row
+    (the CPU baseline) and the Python oracle on the first --link-cpu-frames
+    frames."""
     import ctypes
     from oracle.linker import link_rows
     from tracking_optimizations_amd import _lib, synth
@@ -500,17 +501,26 @@ def bench_link(args) -> int:
 
     cpu = None
     if not args.no_cpu:
+        # the C restatement of the linker (oracle/gt_linker.c), one thread, on
+        # every row of the movie: the CPU baseline of the link step, and a
+        # full-size check of the GPU linker's ids
+        from oracle.oracle import link_rows_c
+        t0 = time.perf_counter()
+        ref = link_rows_c(plain, 5.0, 3)
+        dt = time.perf_counter() - t0
+        same = bool(np.array_equal(ref, ids))
+        cpu = {"value": nf / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"C oracle linker (oracle/gt_linker.c, one thread) on all {nf} frames' "
+                         f"rows ({len(plain)} rows), link only; ids identical to the GPU "
+                         f"linker: {same}"}
+        log(f"CPU C oracle linker: {nf / dt:.0f} frames/s ({dt:.2f} s); ids identical: {same}")
         k = min(args.link_cpu_frames, nf)
         sub = plain[plain[:, 0] < k]
         t0 = time.perf_counter()
-        ref = link_rows(sub, 5.0, 3)
-        dt = time.perf_counter() - t0
-        same = bool(np.array_equal(ref, ids[: len(ref)]))
-        cpu = {"value": k / dt, "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"oracle linker (Python/numpy) on the first {k} frames' rows "
-                         f"({len(sub)} rows), link only; ids identical to the GPU linker: "
-                         f"{same}"}
-        log(f"CPU oracle linker: {k / dt:.1f} frames/s on {k} frames; ids identical: {same}")
+        pref = link_rows(sub, 5.0, 3)
+        pdt = time.perf_counter() - t0
+        log(f"Python oracle linker: {k / pdt:.1f} frames/s on {k} frames; ids identical: "
+            f"{bool(np.array_equal(pref, ids[: len(pref)]))}")
     print(json.dumps({"metric": "frames/s, locate + link (search_range 5, memory 3), "
                                 "2000 x 1024^2 uint16, 3000 particles",
                       "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,

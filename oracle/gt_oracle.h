@@ -122,6 +122,12 @@ int orc_adjoint_particle(const float* frame, int32_t H, int32_t W, const orc_par
 int orc_particle_f64(const double* frame, int32_t H, int32_t W, const orc_params* p,
                      int64_t cand, double* yxm, int32_t* state, int32_t* n_active);
 
+/* link_rust (gpu-tracking/src/lib.rs:202-212, linking.rs) restated in C
+ * (gt_linker.c): ids[i] = particle id of row i of a row-major n x ncols
+ * float32 table (column 0 frame, 2 y, 3 x).  0, or -1 when out of memory. */
+int orc_link_rows(const float* rows, int64_t n, int32_t ncols, float r, int64_t memory,
+                  uint64_t* ids);
+
 #ifdef __cplusplus
 }
 #endif

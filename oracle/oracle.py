@@ -92,8 +92,25 @@ def lib():
                                            ctypes.c_int64, dp, i32p, i32p, dp, i32p]
         L.orc_particle_f64.argtypes = [dp, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.POINTER(OrcParams), ctypes.c_int64, dp, i32p, i32p]
+        L.orc_link_rows.argtypes = [fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
+                                    ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_link_rows.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+def link_rows_c(rows: np.ndarray, search_range: float, memory: int = 0) -> np.ndarray:
+    """The C restatement of link_rust (gt_linker.c): uint64 id per row."""
+    a = np.ascontiguousarray(rows, np.float32)
+    if a.ndim != 2 or a.shape[1] < 4:
+        raise ValueError("rows must be n x >= 4 float32")
+    ids = np.zeros(a.shape[0], np.uint64)
+    rc = lib().orc_link_rows(_ptr(a, ctypes.c_float), a.shape[0], a.shape[1],
+                             float(search_range), int(memory),
+                             ids.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if rc != 0:
+        raise MemoryError("orc_link_rows failed")
+    return ids
 
 
 def _ptr(a: np.ndarray, ctype):

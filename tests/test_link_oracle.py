@@ -245,3 +245,26 @@ def test_neighbours_match_scalar_rule():
             if dist < r2:
                 exp.append((d, dist))
         assert [(d, float(v)) for d, v in lst] == [(d, float(v)) for d, v in exp]
+
+
+@pytest.mark.parametrize("memory", [0, 1, 3])
+def test_c_restatement_matches_python_oracle(memory):
+    """oracle/gt_linker.c (the compiled CPU baseline) against oracle/linker.py
+    on dense blinking Brownian tracks, edge steps included."""
+    from oracle.oracle import link_rows_c
+    rng = np.random.default_rng(70 + memory)
+    pos = rng.uniform(0, 60, (70, 2))
+    rows = []
+    for f in range(25):
+        on = rng.permutation(np.flatnonzero(rng.random(70) >= 0.1))
+        rows += [[f + 2, 1.0, pos[k, 0], pos[k, 1]] for k in on]   # first frame is not 0
+        pos = pos + rng.normal(0, 0.8, pos.shape)
+    rows = np.asarray(rows, np.float32)
+    for r in (2.5, 4.0):
+        assert np.array_equal(link_rows_c(rows, r, memory), link_rows(rows, r, memory))
+    # non-finite coordinates never link; an empty table has no ids
+    bad = rows.copy()
+    bad[5, 2] = np.nan
+    bad[9, 3] = np.inf
+    assert np.array_equal(link_rows_c(bad, 4.0, memory), link_rows(bad, 4.0, memory))
+    assert link_rows_c(np.zeros((0, 4), np.float32), 4.0, memory).shape == (0,)
