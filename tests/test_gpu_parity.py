@@ -250,3 +250,58 @@ def test_row_order_matches_reference(ctx, frames512, case, plain):
     assert np.array_equal(cand, ref_cand)
     assert np.array_equal(rows[:, 0], ref_rows[:, 0])
     assert np.abs(rows[:, 2:4] - ref_rows[:, 2:4]).max() <= POS_TOL
+
+
+def test_full_size_c2_frames(ctx):
+    """Two frames of the benchmark movie itself (2048x2048 uint16, 5000 spots,
+    d = 11, separation 11, smoothing 11, percentile 64), device-resident as in
+    the benchmark: bit-exact candidates and row order, positions and masses
+    within the tolerances, on the batched and the plain path."""
+    import ctypes
+    spec = synth.movie_2048(seed=2, n_frames=2)
+    movie = synth.generate(ctx, spec)
+    host = movie.to_host(np.uint16, (2, 2048, 2048))
+    kw = CASES["c2"]
+    p = make_params(**kw)
+    ref_rows, ref_cand = _oracle_movie(host, kw)
+    print(f"[parity] full-size c2: oracle {len(ref_cand)} rows", flush=True)
+    for flags in (_lib.GT_FRAMES_ON_DEVICE, _lib.GT_FRAMES_ON_DEVICE | _lib.GT_PLAIN):
+        out = ctypes.c_void_p()
+        _lib.check(_lib.load().gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), 2, 2048,
+                                             2048, ctypes.byref(p), flags, ctypes.byref(out)))
+        r = _lib.Result(out)
+        rows, cand = r.rows(), r.candidates()
+        assert np.array_equal(cand, ref_cand), "row order or candidate set differs"
+        _check(rows, cand, ref_rows, ref_cand, f"full-size c2 flags {flags}")
+    movie.free()
+    assert len(ref_cand) > 20000
+
+
+def test_uneven_batch_full_shapes(ctx):
+    """The shapes of the uneven benchmark batch (2048x2048, 512x1024, 256x256)
+    in one gt_locate_batch call from host memory (staged through the pinned
+    chunks): every frame equals the oracle."""
+    import ctypes
+    from tracking_optimizations_amd._lib import GtFrameDesc, Result, check, load
+    big = synth.generate_host(ctx, synth.movie_2048(seed=3, n_frames=2))
+    frames = [big[0], np.ascontiguousarray(big[1, 100:612, 300:1324]),
+              np.ascontiguousarray(big[0, 700:956, 900:1156]), big[1],
+              np.ascontiguousarray(big[1, 1500:1756, 17:273])]
+    kw = CASES["c2"]
+    descs = (GtFrameDesc * len(frames))()
+    for i, f in enumerate(frames):
+        descs[i].data = f.ctypes.data
+        descs[i].h, descs[i].w = f.shape
+        descs[i].frame_index = i
+    out = ctypes.c_void_p()
+    p = make_params(**kw)
+    print("[parity] uneven full shapes", flush=True)
+    check(load().gt_locate_batch(ctx.handle, descs, len(frames), 0, ctypes.byref(p), 0,
+                                 ctypes.byref(out)))
+    r = Result(out)
+    rows, cand = r.rows(), r.candidates()
+    for i, f in enumerate(frames):
+        sel = rows[:, 0] == i
+        ref_rows, ref_cand, _ = oracle.locate_frame(f.astype(np.float32), _oracle_params(kw),
+                                                    frame_index=i)
+        _check(rows[sel], cand[sel], ref_rows, ref_cand, f"uneven-full/{i}/{f.shape}")
