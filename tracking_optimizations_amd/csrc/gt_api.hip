@@ -987,7 +987,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                 ++launches;
             }
             if (adj) {
-                HIP_TRY(launch_adjoint(b, d_geom, ao, (uint32_t)bp.b0, st));
+                HIP_TRY(launch_adjoint(b, g, d_geom, ao, (uint32_t)bp.b0, st));
                 ++launches;
             }
 

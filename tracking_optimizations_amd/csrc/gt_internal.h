@@ -167,7 +167,7 @@ hipError_t launch_raw(const Batch& b, const Geom& g, const Geom* d_geom, hipStre
 // Adjoint (gt_adjoint.hip).
 int adjoint_patch(const Geom& g);
 bool adjoint_supported(const Geom& g);
-hipError_t launch_adjoint(const Batch& b, const Geom* d_geom, const AdjOut& ao,
+hipError_t launch_adjoint(const Batch& b, const Geom& g, const Geom* d_geom, const AdjOut& ao,
                           uint32_t frame_pos0, hipStream_t s);
 hipError_t launch_adjoint_vjp(const float* jac, const int32_t* origin, const uint64_t* frame_rows,
                               uint32_t nframes, const float* grad_rows, float* grad, int S, int H,
