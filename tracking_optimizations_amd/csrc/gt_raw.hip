@@ -43,7 +43,56 @@ __device__ __forceinline__ float raw_pixel(const Frame& fr, int in_type, int y, 
     return ((const float*)fr.in)[i];
 }
 
-constexpr int RAW_ROW_BLOCKS = 64;   // waves per frame, rows strided over them
+constexpr int RAW_ROW_BLOCKS = 512;  // waves per frame, rows strided over them
+
+// Ascending bitonic sort of 64 * VPL values held by one wave, element
+// k * 64 + lane in v[k] of that lane: stages across lanes exchange through
+// shuffles, stages across registers swap within the lane.  Every stage index
+// is a compile-time constant once the loops are unrolled.
+template <int VPL>
+__device__ __forceinline__ void wave_bitonic(float (&v)[VPL], int lane) {
+    constexpr int PP = 64 * VPL;
+#pragma unroll
+    for (int kk = 2; kk <= PP; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j >= 64) {
+                const int jj = j >> 6;
+#pragma unroll
+                for (int k = 0; k < VPL; ++k) {
+                    if ((k & jj) == 0) {
+                        const bool up = ((k * 64 + lane) & kk) == 0;
+                        const float a = v[k], c = v[k ^ jj];
+                        const bool swap = up ? (a > c) : (a < c);
+                        v[k] = swap ? c : a;
+                        v[k ^ jj] = swap ? a : c;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < VPL; ++k) {
+                    const float x = v[k];
+                    const float y = __shfl_xor(x, j, 64);
+                    const bool lower = (lane & j) == 0;
+                    const bool up = ((k * 64 + lane) & kk) == 0;
+                    const float lo = x < y ? x : y, hi = x < y ? y : x;
+                    v[k] = (lower == up) ? lo : hi;
+                }
+            }
+        }
+    }
+}
+
+template <int VPL>
+__device__ __forceinline__ void sort_regs(float* s_v, int lane) {
+    float v[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) v[k] = s_v[k * 64 + lane];
+    wave_bitonic<VPL>(v, lane);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) s_v[k * 64 + lane] = v[k];
+}
 
 __global__ __launch_bounds__(64) void k_raw(Batch b, const Geom* __restrict__ g) {
     extern __shared__ float s_v[];
@@ -89,23 +138,31 @@ __global__ __launch_bounds__(64) void k_raw(Batch b, const Geom* __restrict__ g)
                 if (ok) s_v[n + __popcll(m & lanemask_below(lane))] = v;
                 n += __popcll(m);
             }
-            int P = 1;
+            int P = 64;
             while (P < n) P <<= 1;
             for (int t = n + lane; t < P; t += 64) s_v[t] = __builtin_inff();
             __syncthreads();
-            for (int k = 2; k <= P; k <<= 1) {
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = lane; i < P; i += 64) {
-                        const int l = i ^ j;
-                        if (l > i) {
-                            const float a = s_v[i], c = s_v[l];
-                            if (((i & k) == 0) ? (a > c) : (a < c)) {
-                                s_v[i] = c;
-                                s_v[l] = a;
+            if (P <= 512) {   // in registers (wave-uniform choice)
+                if (P == 64) sort_regs<1>(s_v, lane);
+                else if (P == 128) sort_regs<2>(s_v, lane);
+                else if (P == 256) sort_regs<4>(s_v, lane);
+                else sort_regs<8>(s_v, lane);
+                __syncthreads();
+            } else {
+                for (int k = 2; k <= P; k <<= 1) {
+                    for (int j = k >> 1; j > 0; j >>= 1) {
+                        for (int i = lane; i < P; i += 64) {
+                            const int l = i ^ j;
+                            if (l > i) {
+                                const float a = s_v[i], c = s_v[l];
+                                if (((i & k) == 0) ? (a > c) : (a < c)) {
+                                    s_v[i] = c;
+                                    s_v[l] = a;
+                                }
                             }
                         }
+                        __syncthreads();
                     }
-                    __syncthreads();
                 }
             }
             if (lane == 0) {
@@ -128,7 +185,7 @@ __global__ __launch_bounds__(64) void k_raw(Batch b, const Geom* __restrict__ g)
 }  // namespace
 
 hipError_t launch_raw(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
-    int P = 1;
+    int P = 64;
     while (P < g.raw_nb) P <<= 1;
     const size_t lds = (size_t)std::max(g.raw_ns, g.raw_cols == 3 ? P : 0) * 4 + 16;
     hipLaunchKernelGGL(k_raw, dim3(b.nframes, RAW_ROW_BLOCKS), dim3(64), lds, s, b, d_geom);
