@@ -792,6 +792,36 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
         const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
         const int v1 = x0 - a + lane, v2 = v1 + 64;
         const bool h2 = halo && lane < LC - 64;
+        if (inner) {  // workgroup-uniform
+            // every staged row lies inside the frame: one byte offset per
+            // lane, advanced by four rows per load, and LDS stores at
+            // constant offsets from one address (rows wv + 4q, q < RPW - 1,
+            // always exist; the last group only for the first waves)
+            constexpr int QF = RPW - 1;
+            const bool last = wv + 4 * QF < LR;  // wave-uniform
+            const uint32_t step = 16u * (uint32_t)W;
+            const uint32_t o = 4u * (uint32_t)((y0 - a + wv) * W + v1);
+            float f1[RPW], f2[RPW];
+#pragma unroll
+            for (int q = 0; q < QF; ++q) f1[q] = *(const GLOBAL float*)(Pb + (o + (uint32_t)q * step));
+            if (last) f1[QF] = *(const GLOBAL float*)(Pb + (o + (uint32_t)QF * step));
+            if (h2) {
+#pragma unroll
+                for (int q = 0; q < QF; ++q)
+                    f2[q] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)q * step));
+                if (last) f2[QF] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)QF * step));
+            }
+            float* d = L + wv * LS + lane;
+#pragma unroll
+            for (int q = 0; q < QF; ++q) d[4 * q * LS] = f1[q];
+            if (last) d[4 * QF * LS] = f1[QF];
+            if (h2) {
+#pragma unroll
+                for (int q = 0; q < QF; ++q) d[4 * q * LS + 64] = f2[q];
+                if (last) d[4 * QF * LS + 64] = f2[QF];
+            }
+            return;
+        }
         const uint32_t o1 = 4u * (uint32_t)min(max(v1, 0), W - 1);
         const uint32_t o2 = 4u * (uint32_t)min(max(v2, 0), W - 1);
         float f1[RPW], f2[RPW];
@@ -803,24 +833,15 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
             f1[q] = *(const GLOBAL float*)(Pb + (rb + o1));
             if (h2) f2[q] = *(const GLOBAL float*)(Pb + (rb + o2));
         }
-        if (inner) {  // workgroup-uniform
+        const bool c1 = v1 >= 0 && v1 < W;
+        const bool c2 = v2 >= 0 && v2 < W;
 #pragma unroll
-            for (int q = 0; q < RPW; ++q) {
-                const int r = min(wv + 4 * q, LR - 1);
-                L[r * LS + lane] = f1[q];
-                if (h2) L[r * LS + 64 + lane] = f2[q];
-            }
-        } else {
-            const bool c1 = v1 >= 0 && v1 < W;
-            const bool c2 = v2 >= 0 && v2 < W;
-#pragma unroll
-            for (int q = 0; q < RPW; ++q) {
-                const int r = min(wv + 4 * q, LR - 1);
-                const int u = y0 - a + r;
-                const bool ur = u >= 0 && u < H;  // wave-uniform
-                L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
-                if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
-            }
+        for (int q = 0; q < RPW; ++q) {
+            const int r = min(wv + 4 * q, LR - 1);
+            const int u = y0 - a + r;
+            const bool ur = u >= 0 && u < H;  // wave-uniform
+            L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
+            if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
         }
     };
     stage_tile(true);
@@ -886,15 +907,17 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     const uint64_t colmask = __builtin_amdgcn_ballot_w64((v < W) & (v - dp.band >= 0) &
                                                          (v + dp.band < W));
     const bool diw = dp.diw != 0;
+    // rows m of the wave's run that can hold a candidate: band <= u0 + m <= H - 1 - band
+    const int u0 = y0 + kw * N;
+    const int rlo = max(dp.band - u0, 0), rhi = min(H - 1 - dp.band - u0, N - 1);
+    const uint32_t rowbits = rhi >= rlo ? ((2u << rhi) - (1u << rlo)) : 0u;
     uint64_t bal[N];
     uint32_t mycnt = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        const int u = y0 + kw * N + m;
-        const bool rowok = (u < H) & (u - dp.band >= 0) & (u + dp.band < H);  // uniform
         const uint64_t ge = __builtin_amdgcn_ballot_w64(center[m] >= cm[m]);
         const uint64_t nz = diw ? __builtin_amdgcn_ballot_w64(center[m] != 0) : ~0ull;
-        bal[m] = rowok ? (colmask & ge & nz) : 0ull;
+        bal[m] = (rowbits >> m) & 1u ? (colmask & ge & nz) : 0ull;
         mycnt = (c == m) ? (uint32_t)__popcll(bal[m]) : mycnt;
     }
     if (c < N) s_rowcnt[kw * N + c] = mycnt;
