@@ -23,6 +23,7 @@
 // the host has verified, for every significand, that it rounds exactly like
 // IEEE division; otherwise they are IEEE divisions.
 #include <algorithm>
+#include <type_traits>
 
 #include "gt_internal.h"
 
@@ -266,9 +267,10 @@ void k_bandpass(Batch b, K1Params kp) {
     static_assert(R <= 8, "taps must stay inside the staged halo");
     // per parity: boxcar copy (mirrored) then Gaussian copy (zero outside)
     __shared__ __attribute__((aligned(16))) float buf[2][2][K1_ROWBUF];
-    // two 16-bit bins per word; a work item adds at most
-    // K1_STRIP * seg < 65536 to one bin (checked by the host)
-    __shared__ uint32_t s_hist[HIST_BINS / 2 + 64];
+    // two 16-bit bins per word (word i: bins 2i + 1 and 2i + 2; zeros are
+    // counted apart); a work item adds at most K1_STRIP * seg < 65536 to one
+    // bin (checked by the host)
+    __shared__ uint32_t s_hist[HIST_BINS / 2];
 
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t f = b.k1_frame[item];
@@ -293,7 +295,7 @@ void k_bandpass(Batch b, K1Params kp) {
     const int rbase = min(max(4 * t - 8, 0), K1_ROWBUF - 20);
 
     if (kp.hist) {
-        for (int i = t; i < HIST_BINS / 2 + 64; i += 64) s_hist[i] = 0;
+        for (int i = t; i < HIST_BINS / 2; i += 64) s_hist[i] = 0;
     }
 
     float w[NT];
@@ -322,114 +324,123 @@ void k_bandpass(Batch b, K1Params kp) {
         if (vr0 + s < vr1) pm[s] = ld_chunk<T>(row_ptr(vr0 + s), csrc);
     }
 
-    int parity = 0;
-    for (int base = vr0; base < vr1; base += NT) {
-#pragma unroll
-        for (int s = 0; s < NT; ++s) {
-            const int vr = base + s;
-            if (vr < vr1) {  // wave-uniform
-            float* bb_ = buf[parity][0];
-            float* bg_ = buf[parity][1];
-            parity ^= 1;
-            // rows above and below the frame are zero in the Gaussian copy
-            // (skipped taps) and mirrored in the boxcar copy
-            const bool row_in = (vr >= 0) && (vr < H);
-            {
-                f4v m = unpack_chunk<T>(pm[s]);
-                if (border || !row_in) {
-                    if (crev) m = m.wzyx;
-                    *(f4v*)(bb_ + 4 * t) = m;
-                    if (coob || !row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
-                    *(f4v*)(bg_ + 4 * t) = m;
-                } else {
-                    *(f4v*)(bb_ + 4 * t) = m;
-                    *(f4v*)(bg_ + 4 * t) = m;
-                }
-            }
-            if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(row_ptr(vr + PF), csrc);
-            __syncthreads();
-
-            // boxcar row pass on the mirrored copy (integer sums: exact)
-            float v[20];
-            lds_ld20(bb_ + rbase, v);
-            float tb[4];
-            {
-                float bx = v[8 - R];
-#pragma unroll
-                for (int i = -R + 1; i <= R; ++i) bx = bx + v[8 + i];
-                tb[0] = bx;
-#pragma unroll
-                for (int q = 1; q < 4; ++q) {
-                    bx = (bx + v[8 + q + R]) - v[8 + q - 1 - R];
-                    tb[q] = bx;
-                }
-            }
-            // Gaussian row pass: zeros outside the frame (skipped taps); the
-            // second copy is read in every strip so no registers are merged
-            float u[20];
-            lds_ld20(bg_ + rbase, u);
-            float tg[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                float gs = 0.f;
-#pragma unroll
-                for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
-                tg[q] = gs;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float gq = cdiv_t(tg[q], kp.gn, kp.rgn, FAST);
-                vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
-                rg[s][q] = gq;
-                rb[s][q] = tb[q];
-            }
-
-            if (vr >= y0 + R) {
-                const int o = vr - R;
-                float out[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float gsum = 0.f;
-#pragma unroll
-                    for (int j = 0; j < NT; ++j)
-                        gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
-                    const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
-                                      cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
-                    out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
-                }
-                if (outl) {
-                    *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cc) =
-                        make_float4(out[0], out[1], out[2], out[3]);
-                }
-                if (kp.hist && outl) {
-                    // key (bits >> 19) + 1 for positive values (finite and
-                    // >= +0, so no clamp is needed); zero pixels are counted
-                    // in a word of the lane's own, away from the shared bins
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const uint32_t bits = __float_as_uint(out[q]);
-                        const uint32_t k = (bits >> 19) + 1u;
-                        const bool nz = bits != 0u;
-                        const uint32_t word = nz ? (k >> 1) : (uint32_t)(HIST_BINS / 2 + t);
-                        const uint32_t val = (nz && (k & 1u)) ? 0x10000u : 1u;
-                        atomicAdd(&s_hist[word], val);
+    uint32_t zc = 0;  // zero outputs of this work item (wave-uniform)
+    const uint64_t outl_m = __builtin_amdgcn_ballot_w64(outl);
+    // the row loop, specialised on whether any chunk of the strip lies
+    // outside the frame (mirrored / zeroed staging) or none does
+    auto rows = [&](auto border_c) {
+        constexpr bool BORDER = decltype(border_c)::value;
+        int parity = 0;
+        for (int base = vr0; base < vr1; base += NT) {
+    #pragma unroll
+            for (int s = 0; s < NT; ++s) {
+                const int vr = base + s;
+                if (vr < vr1) {  // wave-uniform
+                float* bb_ = buf[parity][0];
+                float* bg_ = buf[parity][1];
+                parity ^= 1;
+                // rows above and below the frame are zero in the Gaussian copy
+                // (skipped taps) and mirrored in the boxcar copy
+                const bool row_in = (vr >= 0) && (vr < H);
+                {
+                    f4v m = unpack_chunk<T>(pm[s]);
+                    if constexpr (BORDER) {
+                        if (crev) m = m.wzyx;
+                        *(f4v*)(bb_ + 4 * t) = m;
+                        if (coob || !row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
+                        *(f4v*)(bg_ + 4 * t) = m;
+                    } else {
+                        // every chunk of an interior strip is its own
+                        *(f4v*)(bb_ + 4 * t) = m;
+                        if (!row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
+                        *(f4v*)(bg_ + 4 * t) = m;
                     }
                 }
+                if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(row_ptr(vr + PF), csrc);
+                __syncthreads();
+
+                // boxcar row pass on the mirrored copy (integer sums: exact)
+                float v[20];
+                lds_ld20(bb_ + rbase, v);
+                float tb[4];
+                {
+                    float bx = v[8 - R];
+    #pragma unroll
+                    for (int i = -R + 1; i <= R; ++i) bx = bx + v[8 + i];
+                    tb[0] = bx;
+    #pragma unroll
+                    for (int q = 1; q < 4; ++q) {
+                        bx = (bx + v[8 + q + R]) - v[8 + q - 1 - R];
+                        tb[q] = bx;
+                    }
+                }
+                // Gaussian row pass: zeros outside the frame (skipped taps); the
+                // second copy is read in every strip so no registers are merged
+                float u[20];
+                lds_ld20(bg_ + rbase, u);
+                float tg[4];
+    #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float gs = 0.f;
+    #pragma unroll
+                    for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
+                    tg[q] = gs;
+                }
+    #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float gq = cdiv_t(tg[q], kp.gn, kp.rgn, FAST);
+                    vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
+                    rg[s][q] = gq;
+                    rb[s][q] = tb[q];
+                }
+
+                if (vr >= y0 + R) {
+                    const int o = vr - R;
+                    float out[4];
+    #pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float gsum = 0.f;
+    #pragma unroll
+                        for (int j = 0; j < NT; ++j)
+                            gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
+                        const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
+                                          cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
+                        out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
+                    }
+                    if (outl) {
+                        *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cc) =
+                            make_float4(out[0], out[1], out[2], out[3]);
+                    }
+                    if (kp.hist) {  // wave-uniform: the zero count is a ballot over all lanes
+                        // bin (bits >> 19) + 1 for positive values (finite and
+                        // >= +0, so no clamp is needed): word bits >> 20, half
+                        // bit 19; zero pixels (most of a frame) are counted by
+                        // a ballot instead of contending for one word
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const uint32_t bits = __float_as_uint(out[q]);
+                            zc += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(bits == 0u) & outl_m);
+                            if (outl && bits != 0u)
+                                atomicAdd(&s_hist[bits >> 20], 1u << ((bits >> 15) & 16u));
+                        }
+                    }
+                }
+                }  // vr < vr1
             }
-            }  // vr < vr1
         }
-    }
+    };
+    if (border) rows(std::true_type{});
+    else rows(std::false_type{});
 
     if (kp.hist) {
         __syncthreads();
         uint32_t* gh = b.hist1 + (size_t)f * HIST_BINS;
         for (int i = t; i < HIST_BINS / 2; i += 64) {
             const uint32_t c = s_hist[i];
-            if (c & 0xffffu) atomicAdd(&gh[2 * i], c & 0xffffu);
-            if (c >> 16) atomicAdd(&gh[2 * i + 1], c >> 16);
+            if (c & 0xffffu) atomicAdd(&gh[2 * i + 1], c & 0xffffu);
+            if ((c >> 16) && 2 * i + 2 < HIST_BINS) atomicAdd(&gh[2 * i + 2], c >> 16);
         }
-        const uint32_t z = wave_sum_u(s_hist[HIST_BINS / 2 + t]);
-        if (t == 0 && z) atomicAdd(&gh[0], z);
+        if (t == 0 && zc) atomicAdd(&gh[0], zc);
     }
 }
 
