@@ -8,7 +8,6 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <deque>
 #include <functional>
 #include <future>
 #include <map>
@@ -569,9 +568,16 @@ struct LinkPipeline {
     const gt_params* p;
     LinkState* state;
     std::future<std::pair<gt_status, std::string>> tail;
-    std::deque<std::vector<uint64_t>> ids;
     LinkStats total{};
-    LinkPipeline(gt_context* c, const gt_params* prm, uint64_t n_frames) : ctx(c), p(prm) {
+    // the result rows (particle column at index 4, `stride` floats per row),
+    // grown by the caller under `mu`; the worker writes each chunk's ids
+    // there under the same lock
+    std::vector<float>* data;
+    std::mutex* mu;
+    uint32_t stride;
+    LinkPipeline(gt_context* c, const gt_params* prm, uint64_t n_frames, std::vector<float>* d,
+                 std::mutex* m, uint32_t st)
+        : ctx(c), p(prm), data(d), mu(m), stride(st) {
         const uint64_t mem = prm->has_memory ? prm->memory : 0;
         state = link_state_create((uint32_t)std::min<uint64_t>(mem, std::max<uint64_t>(n_frames, 1)));
         if (!ctx->link_cache) ctx->link_cache = link_cache_create();
@@ -580,9 +586,10 @@ struct LinkPipeline {
         if (tail.valid()) tail.wait();
         link_state_destroy(state);
     }
-    // rows: the chunk's rows (nc columns, frame order), frames: its frames
+    // rows: the chunk's rows (nc columns, frame order), frames: its frames,
+    // r0: the chunk's first row in the result
     gt_status submit(const std::vector<FrameIn>& frames, const float* rows, uint64_t nrows,
-                     uint32_t nc) {
+                     uint32_t nc, uint64_t r0) {
         std::vector<uint32_t> so(frames.size() + 1, 0);
         std::vector<float> ys(nrows), xs(nrows);
         uint64_t r = 0;
@@ -595,12 +602,10 @@ struct LinkPipeline {
             so[f + 1] = (uint32_t)r;
         }
         if (r != nrows) return fail(GT_E_INTERNAL, "link: rows are not in frame order");
-        ids.emplace_back(nrows);
-        uint64_t* out = ids.back().data();
         auto prev = std::move(tail);
         tail = std::async(std::launch::async,
                           [this, prev = std::move(prev), ys = std::move(ys), xs = std::move(xs),
-                           so = std::move(so), out]() mutable -> std::pair<gt_status, std::string> {
+                           so = std::move(so), nrows, r0]() mutable -> std::pair<gt_status, std::string> {
                               if (prev.valid()) {
                                   auto pr = prev.get();
                                   if (pr.first != GT_OK) return pr;
@@ -611,9 +616,10 @@ struct LinkPipeline {
                               std::string err;
                               hipError_t e = link_cache_stream(ctx->link_cache, &st);
                               LinkStats ls;
+                              std::vector<uint64_t> ids(nrows);
                               if (e == hipSuccess)
                                   e = link_chunk(st, ctx->link_cache, state, ys.data(), xs.data(), so,
-                                                 p->search_range, out, &ls, err);
+                                                 p->search_range, ids.data(), &ls, err);
                               if (e != hipSuccess)
                                   return {e == hipErrorOutOfMemory ? GT_E_OOM
                                           : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP,
@@ -625,32 +631,22 @@ struct LinkPipeline {
                               total.ids = ls.ids;
                               total.search_ms += ls.search_ms;
                               total.solve_ms += ls.solve_ms;
+                              {
+                                  std::lock_guard<std::mutex> lk(*mu);
+                                  float* d = data->data() + r0 * stride + 4;
+                                  for (uint64_t i = 0; i < nrows; ++i) d[i * stride] = (float)ids[i];
+                              }
                               return {GT_OK, {}};
                           });
         return GT_OK;
     }
-    // waits for the last chunk, then adds the particle column (after x) to res
-    gt_status finish(gt_result* res) {
+    // waits for the last chunk (every id is then in place)
+    gt_status finish() {
         if (tail.valid()) {
             auto r = tail.get();
             if (r.first != GT_OK) return fail(r.first, "%s", r.second.c_str());
         }
         ctx->last_link = total;
-        const uint32_t nc = res->ncols;
-        std::vector<float> out(res->rows * (nc + 1));
-        uint64_t i = 0;
-        for (const auto& v : ids)
-            for (uint64_t id : v) {
-                const float* src = res->data.data() + i * nc;
-                float* dst = out.data() + i * (nc + 1);
-                for (uint32_t c = 0; c < 4; ++c) dst[c] = src[c];
-                dst[4] = (float)id;
-                for (uint32_t c = 4; c < nc; ++c) dst[c + 1] = src[c];
-                ++i;
-            }
-        if (i != res->rows) return fail(GT_E_INTERNAL, "link: row count mismatch");
-        res->data.swap(out);
-        res->ncols = nc + 1;
         return GT_OK;
     }
 };
@@ -1504,8 +1500,14 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         res->ncols = nc;
     }
     gt_timing total{};
+    // with linking, the rows are stored with the particle column (index 4,
+    // after x) in place, and the link worker fills it in chunk by chunk
+    const uint32_t gap = p->has_search_range ? 1u : 0u;
+    const uint32_t stride = res->ncols + gap;
+    std::mutex data_mu;  // res->data grows here while the worker writes ids into it
     std::unique_ptr<LinkPipeline> lp;
-    if (p->has_search_range) lp.reset(new LinkPipeline(ctx, p, frames.size()));
+    if (p->has_search_range)
+        lp.reset(new LinkPipeline(ctx, p, frames.size(), &res->data, &data_mu, stride));
     if (host && nchunks) {
         const Status r0 = fill_chunk(0);
         if (r0.first != GT_OK) return fail(r0.first, "%s", r0.second.c_str());
@@ -1522,15 +1524,32 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         if (st != GT_OK) return st;
         std::unique_ptr<gt_result> partp(part);
         if (rs.first != GT_OK) return fail(rs.first, "%s", rs.second.c_str());
-        if (lp) GT_TRY(lp->submit(fr, part->data.data(), part->rows, part->ncols));
-        if (c == 0 && nchunks > 1) {   // room for all chunks at the first one's density
-            const size_t est = (size_t)((double)part->rows * frames.size() / fr.size() * 1.1);
-            res->data.reserve(est * part->ncols);
-            res->cand.reserve(est);
+        if (part->ncols + gap != stride) return fail(GT_E_INTERNAL, "locate: column count mismatch");
+        const uint64_t r0 = res->rows;
+        {
+            std::lock_guard<std::mutex> lk(data_mu);
+            if (c == 0 && nchunks > 1) {   // room for all chunks at the first one's density
+                const size_t est = (size_t)((double)part->rows * frames.size() / fr.size() * 1.1);
+                res->data.reserve(est * stride);
+                res->cand.reserve(est);
+            }
+            if (!gap) {
+                res->data.insert(res->data.end(), part->data.begin(), part->data.end());
+            } else {
+                const uint32_t nc = part->ncols;
+                const size_t old = res->data.size();
+                res->data.resize(old + part->rows * stride);
+                float* dst = res->data.data() + old;
+                const float* src = part->data.data();
+                for (uint64_t i = 0; i < part->rows; ++i, dst += stride, src += nc) {
+                    for (uint32_t k = 0; k < 4; ++k) dst[k] = src[k];
+                    for (uint32_t k = 4; k < nc; ++k) dst[k + 1] = src[k];
+                }
+            }
         }
-        res->data.insert(res->data.end(), part->data.begin(), part->data.end());
         res->cand.insert(res->cand.end(), part->cand.begin(), part->cand.end());
         res->rows += part->rows;
+        if (lp) GT_TRY(lp->submit(fr, part->data.data(), part->rows, part->ncols, r0));
         total.device_ms += ctx->last.device_ms;
         total.frames += ctx->last.frames;
         total.pixels += ctx->last.pixels;
@@ -1538,7 +1557,8 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
         total.rows += ctx->last.rows;
         total.launches += ctx->last.launches;
     }
-    if (lp) GT_TRY(lp->finish(res.get()));
+    if (lp) GT_TRY(lp->finish());
+    res->ncols = stride;
     {
         const char* names[16];
         const char* dtypes[16];
