@@ -305,3 +305,17 @@ def test_binary32_error_against_binary64():
     assert pos < 1e-3, pos
     assert mass < 1e-5, mass
     assert abs(t32 - t64) <= 1e-5 * max(1.0, abs(t64))
+    # every characterisation column (frame, mass, y, x, Rg, ecc, signal,
+    # Rg_gpu, raw_mass_gpu, signal_gpu, ecc_gpu): relative error for the
+    # sizes and sums, absolute for the eccentricities (which approach 0)
+    a, b = r32[i32], r64[i64]
+    rel = lambda j: (np.abs(a[:, j] - b[:, j]) / np.abs(b[:, j])).max()
+    ab = lambda j: np.abs(a[:, j] - b[:, j]).max()
+    assert a.shape[1] == 11
+    for j in (4, 7):          # Rg, Rg_gpu
+        assert rel(j) < 1e-4, (j, rel(j))
+    for j in (6, 9):          # signal, signal_gpu
+        assert rel(j) < 1e-4, (j, rel(j))
+    assert rel(8) < 1e-5, rel(8)   # raw_mass_gpu
+    for j in (5, 10):         # ecc, ecc_gpu
+        assert ab(j) < 1e-3, (j, ab(j))
