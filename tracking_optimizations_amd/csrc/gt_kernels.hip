@@ -1725,8 +1725,8 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 constexpr int DD_CAP = 256;   // staged records per wave (more fall back to global scans)
 
 __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__ g) {
-    __shared__ float s_m[4][DD_CAP], s_y[4][DD_CAP], s_x[4][DD_CAP];
-    __shared__ int32_t s_i[4][DD_CAP];
+    // staged records, one 16-byte entry each: mass, y, x, raster index (bits)
+    __shared__ float4 s_rec[4][DD_CAP];
     __shared__ uint32_t s_kr[4][TILE_H];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
@@ -1825,10 +1825,8 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                 const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
                 const uint32_t l2 = slots[s2];
                 const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
-                s_m[wave][r] = m;
-                s_y[wave][r] = y;
-                s_x[wave][r] = x;
-                s_i[wave][r] = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
+                const int32_t ri = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
+                s_rec[wave][r] = make_float4(m, y, x, __int_as_float(ri));
             }
             S += nt;
         }
@@ -1852,30 +1850,22 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
         const bool valid = own && ok(m, slot0 + i);
         bool keep = valid;
         if (staged) {
-            // one own candidate at a time, staged records across the lanes
-            uint64_t vm = __ballot(valid);
-            uint64_t dropped = 0;
-            while (vm) {
-                const int src = __ffsll((long long)vm) - 1;
-                vm &= vm - 1;
-                const float om = __shfl(m, src, 64), oy = __shfl(y, src, 64);
-                const float ox = __shfl(x, src, 64);
-                const int32_t oi = __shfl(idx, src, 64);
-                bool drop = false;
-                for (uint32_t jb = 0; jb < S; jb += 64) {
-                    const uint32_t j = jb + lane;
-                    if (j < S) {
-                        const float nm = s_m[wave][j];
-                        const float dy = s_y[wave][j] - oy, dx = s_x[wave][j] - ox;
-                        float d = dy * dy;
-                        d = d + dx * dx;
-                        const int32_t nidx = s_i[wave][j];
-                        drop |= d < r2 && nidx != oi && (om < nm || (om == nm && oi > nidx));
-                    }
+            // every lane tests its own candidate against the staged records
+            // in turn (one broadcast 16-byte LDS read per record; the reads
+            // do not depend on each other, so they pipeline)
+            bool drop = false;
+            if (__ballot(valid)) {  // wave-uniform
+#pragma unroll 4
+                for (uint32_t j = 0; j < S; ++j) {
+                    const float4 rr = s_rec[wave][j];
+                    const float dy = rr.y - y, dx = rr.z - x;
+                    float d = dy * dy;
+                    d = d + dx * dx;
+                    const int32_t nidx = __float_as_int(rr.w);
+                    drop |= d < r2 && nidx != idx && (m < rr.x || (m == rr.x && idx > nidx));
                 }
-                if (__ballot(drop)) dropped |= 1ull << src;
             }
-            keep = valid && !((dropped >> lane) & 1ull);
+            keep = valid && !drop;
         } else if (__ballot(keep)) {
             for (int ny = y_lo; ny <= y_hi; ++ny) {
                 for (int nx = x_lo; nx <= x_hi; ++nx) {
