@@ -1805,8 +1805,8 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
             const uint32_t j = jb + lane;
             // tile of record j: the last q whose first record is <= j
             int q = 0;
-            for (int qq = 1; qq < ntl; ++qq)
-                if (__shfl(exq, qq, 64) <= j) q = qq;
+            for (int qq = 1; qq < ntl; ++qq)  // qq is wave-uniform: a scalar read of lane qq
+                if ((uint32_t)__builtin_amdgcn_readlane((int)exq, qq) <= j) q = qq;
             const uint32_t it2 = __shfl(myit, q, 64);
             const uint32_t i2 = j - __shfl(exq, q, 64);
             const size_t s2 = (size_t)it2 * TILE_PX + i2;
