@@ -2066,19 +2066,27 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
     uint32_t before = 0;  // kept rows of this tile before the chunk
     for (uint32_t ib = 0; ib < cnt; ib += 64) {
         const uint32_t i = ib + lane;
-        const bool k = (i < cnt) && b.keep[slot0 + i];
+        // the keep flag, the slot and the position record of every candidate
+        // of the chunk are loaded together (all in bounds for i < cnt); only
+        // the row offset waits for the slot
+        const bool in = i < cnt;
+        const size_t s = slot0 + (in ? i : 0u);
+        const bool kf = b.keep[s];
+        const uint32_t l = b.slots[s];
+        const float rm = b.rec[REC_MASS * cap + s];
+        const float ryv = b.rec[REC_Y * cap + s];
+        const float rxv = b.rec[REC_X * cap + s];
+        const bool k = in && kf;
         const uint64_t msk = __ballot(k);
         if (k) {
-            const uint32_t l = b.slots[slot0 + i];
             const uint64_t r = fbase + b.rowoff[(size_t)item * TILE_H + l / TILE_W] + before +
                                (uint64_t)__popcll(msk & lanemask_lt(lane));
             if (r < b.rows_cap) {
                 float* o = b.rows + r * (uint64_t)nc;
-                const size_t s = slot0 + i;
                 o[0] = (float)fr.frame_index;
-                o[1] = b.rec[REC_MASS * cap + s];
-                o[2] = b.rec[REC_Y * cap + s];
-                o[3] = b.rec[REC_X * cap + s];
+                o[1] = rm;
+                o[2] = ryv;
+                o[3] = rxv;
                 int c = 4;
                 if (g->cpu_processed) {
                     o[c++] = b.rec[REC_RG * cap + s];
