@@ -2,6 +2,8 @@
 // persistent device context, batch planning and the C ABI of
 // include/gpu_tracking.h.
 #include <algorithm>
+#include <cstdlib>
+#include <new>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -374,11 +376,82 @@ struct gt_context {
     size_t pinned_bytes = 0;
 };
 
+// Host result buffers.  Blocks of 1 MiB and more are kept when a result is
+// freed (up to 256 MiB in all) and handed to the next result that fits, so
+// calls in a loop write their rows into memory whose pages are already
+// mapped instead of faulting in fresh pages on every call.
+class HostBlockCache {
+  public:
+    static void* get(size_t bytes) {
+        if (bytes >= kMin) {
+            std::lock_guard<std::mutex> lk(mu());
+            auto& v = blocks();
+            size_t best = v.size();
+            for (size_t i = 0; i < v.size(); ++i)
+                if (v[i].second >= bytes && v[i].second <= 2 * bytes &&
+                    (best == v.size() || v[i].second < v[best].second))
+                    best = i;
+            if (best != v.size()) {
+                void* p = v[best].first;
+                total() -= v[best].second;
+                v.erase(v.begin() + (std::ptrdiff_t)best);
+                return p;
+            }
+        }
+        void* p = std::malloc(bytes ? bytes : 1);
+        if (!p) throw std::bad_alloc();
+        return p;
+    }
+    // bytes: the size the block was requested with (a reused block may be larger)
+    static void put(void* p, size_t bytes) {
+        if (!p) return;
+        if (bytes < kMin || bytes > kMax) {
+            std::free(p);
+            return;
+        }
+        std::lock_guard<std::mutex> lk(mu());
+        auto& v = blocks();
+        v.emplace_back(p, bytes);
+        total() += bytes;
+        while (total() > kMax) {   // drop the oldest
+            total() -= v.front().second;
+            std::free(v.front().first);
+            v.erase(v.begin());
+        }
+    }
+
+  private:
+    static constexpr size_t kMin = (size_t)1 << 20, kMax = (size_t)256 << 20;
+    // never destroyed: results may be freed during process teardown
+    static std::mutex& mu() { static std::mutex* m = new std::mutex; return *m; }
+    static std::vector<std::pair<void*, size_t>>& blocks() {
+        static auto* v = new std::vector<std::pair<void*, size_t>>;
+        return *v;
+    }
+    static size_t& total() { static size_t t = 0; return t; }
+};
+
+template <class T>
+struct HostAlloc {
+    using value_type = T;
+    HostAlloc() = default;
+    template <class U>
+    HostAlloc(const HostAlloc<U>&) {}
+    T* allocate(size_t n) { return (T*)HostBlockCache::get(n * sizeof(T)); }
+    void deallocate(T* p, size_t n) { HostBlockCache::put(p, n * sizeof(T)); }
+    template <class U>
+    bool operator==(const HostAlloc<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const HostAlloc<U>&) const { return false; }
+};
+template <class T>
+using HostVec = std::vector<T, HostAlloc<T>>;
+
 struct gt_result {
     uint64_t rows = 0;
     uint32_t ncols = 0;
-    std::vector<float> data;
-    std::vector<int64_t> cand;
+    HostVec<float> data;
+    HostVec<int64_t> cand;
     const float* ddata = nullptr;
     std::vector<std::string> names, dtypes;
 };
@@ -393,8 +466,8 @@ struct gt_adjoint {
     uint64_t nframes = 0;
     uint32_t h = 0, w = 0;     // frame shape (0 when the call had mixed shapes)
     bool jacobian = false;
-    std::vector<float> sigma;
-    std::vector<int32_t> origin, rowpos;
+    HostVec<float> sigma;
+    HostVec<int32_t> origin, rowpos;
     DevBuf d_jac, d_origin, d_frame_rows;
     ~gt_adjoint() {
         delete fwd;
@@ -547,7 +620,7 @@ gt_status link_result(gt_context* ctx, const std::vector<FrameIn>& frames, const
     if (e != hipSuccess)
         return fail(e == hipErrorOutOfMemory ? GT_E_OOM
                     : e == hipErrorInvalidValue ? GT_E_INVALID : GT_E_HIP, "%s", err.c_str());
-    std::vector<float> out(n * (nc + 1));
+    HostVec<float> out(n * (nc + 1));
     for (uint64_t i = 0; i < n; ++i) {
         const float* src = res->data.data() + i * nc;
         float* dst = out.data() + i * (nc + 1);
@@ -572,10 +645,10 @@ struct LinkPipeline {
     // the result rows (particle column at index 4, `stride` floats per row),
     // grown by the caller under `mu`; the worker writes each chunk's ids
     // there under the same lock
-    std::vector<float>* data;
+    HostVec<float>* data;
     std::mutex* mu;
     uint32_t stride;
-    LinkPipeline(gt_context* c, const gt_params* prm, uint64_t n_frames, std::vector<float>* d,
+    LinkPipeline(gt_context* c, const gt_params* prm, uint64_t n_frames, HostVec<float>* d,
                  std::mutex* m, uint32_t st)
         : ctx(c), p(prm), data(d), mu(m), stride(st) {
         const uint64_t mem = prm->has_memory ? prm->memory : 0;
