@@ -332,7 +332,7 @@ void k_bandpass(Batch b, K1Params kp) {
         constexpr bool BORDER = decltype(border_c)::value;
         int parity = 0;
         for (int base = vr0; base < vr1; base += NT) {
-    #pragma unroll
+#pragma unroll
             for (int s = 0; s < NT; ++s) {
                 const int vr = base + s;
                 if (vr < vr1) {  // wave-uniform
@@ -365,10 +365,10 @@ void k_bandpass(Batch b, K1Params kp) {
                 float tb[4];
                 {
                     float bx = v[8 - R];
-    #pragma unroll
+#pragma unroll
                     for (int i = -R + 1; i <= R; ++i) bx = bx + v[8 + i];
                     tb[0] = bx;
-    #pragma unroll
+#pragma unroll
                     for (int q = 1; q < 4; ++q) {
                         bx = (bx + v[8 + q + R]) - v[8 + q - 1 - R];
                         tb[q] = bx;
@@ -379,14 +379,14 @@ void k_bandpass(Batch b, K1Params kp) {
                 float u[20];
                 lds_ld20(bg_ + rbase, u);
                 float tg[4];
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     float gs = 0.f;
-    #pragma unroll
+#pragma unroll
                     for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
                     tg[q] = gs;
                 }
-    #pragma unroll
+#pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const float gq = cdiv_t(tg[q], kp.gn, kp.rgn, FAST);
                     vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
@@ -397,10 +397,10 @@ void k_bandpass(Batch b, K1Params kp) {
                 if (vr >= y0 + R) {
                     const int o = vr - R;
                     float out[4];
-    #pragma unroll
+#pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         float gsum = 0.f;
-    #pragma unroll
+#pragma unroll
                         for (int j = 0; j < NT; ++j)
                             gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
                         const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
