@@ -305,3 +305,24 @@ def test_uneven_batch_full_shapes(ctx):
         ref_rows, ref_cand, _ = oracle.locate_frame(f.astype(np.float32), _oracle_params(kw),
                                                     frame_index=i)
         _check(rows[sel], cand[sel], ref_rows, ref_cand, f"uneven-full/{i}/{f.shape}")
+
+
+def test_host_result_blocks_reused(ctx):
+    """Host results of 1 MiB and more are kept after they are freed and reused
+    by the next result that fits: a result written into a reused block (here
+    four 2048x2048 frames with the characterisation columns, about 1.6 MB of
+    rows, after a slightly larger result was freed) equals the oracle, and a
+    repeat of the same call into the block freed by the first is identical."""
+    spec = synth.movie_2048(seed=2, n_frames=4)
+    host = synth.generate_host(ctx, spec)
+    big = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, characterize=True)
+    kw = dict(diameter=11, separation=13, smoothing_size=11, percentile=64, characterize=True)
+    r0 = locate(host, make_params(**big), ctx=ctx)[0]
+    assert r0.nbytes >= (1 << 20)
+    del r0
+    rows, cand, _, _ = locate(host, make_params(**kw), ctx=ctx)
+    assert rows.nbytes >= (1 << 20)
+    ref_rows, ref_cand = _oracle_movie(host, kw)
+    _check(rows, cand, ref_rows, ref_cand, "reused host blocks")
+    rows2, cand2, _, _ = locate(host, make_params(**kw), ctx=ctx)
+    assert np.array_equal(rows, rows2) and np.array_equal(cand, cand2)
