@@ -89,12 +89,13 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
-    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link", "file"],
+    ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link", "file", "raw"],
                     default="movie",
                     help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames; "
                          "adjoint: 256 x 512^2 localisation errors and Jacobians; "
                          "link: locate + link, 2000 x 1024^2, 3000 blinking particles; "
-                         "file: batch_file on a 256 x 2048^2 uint16 TIFF")
+                         "file: batch_file on a 256 x 2048^2 uint16 TIFF; "
+                         "raw: raw signal columns on 200 frames of the headline movie")
     ap.add_argument("--link-cpu-frames", type=int, default=150,
                     help="frames of the link workload also linked by the Python oracle")
     args = ap.parse_args()
@@ -115,6 +116,8 @@ def main() -> int:
         return bench_link(args)
     if args.workload == "file":
         return bench_file(args)
+    if args.workload == "raw":
+        return bench_raw(args)
 
     kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
     if args.no_percentile:
@@ -536,6 +539,60 @@ def bench_link(args) -> int:
                       "locate_only_frames_per_s": loc_fps,
                       "link_rust_only_frames_per_s": nf / link_el,
                       "cpu_baseline": cpu}), flush=True)
+    movie.free()
+    return 0
+
+
+def bench_raw(args) -> int:
+    """Raw signal columns (raw_mass, raw_bg_median, raw_mass_corrected) on
+    200 frames of the headline movie (2048x2048 uint16, d = 11, separation 11,
+    smoothing 11, percentile 64): signal disk radius 3, background annulus
+    radius 8 outside a gap of 1, frames and rows resident in HBM; beside it
+    the same call without the background and without the raw columns."""
+    import ctypes
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import make_params
+    ctx = _lib.Context(0)
+    L = _lib.load()
+    nf = 200 if args.frames == 1000 else args.frames
+    movie = synth.generate(ctx, synth.movie_2048(seed=2, n_frames=nf))
+    ctx.synchronize()
+    base = dict(separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    variants = [("locate", {}), ("signal", dict(sig_radius=3.0)),
+                ("signal + background", dict(sig_radius=3.0, bg_radius=8.0, gap_radius=1.0))]
+    flags = _lib.GT_FRAMES_ON_DEVICE | _lib.GT_RESULT_ON_DEVICE
+    res = {}
+    for name, kw in variants:
+        p = make_params(11, **base, **kw)
+
+        def run():
+            out = ctypes.c_void_p()
+            _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), nf, 2048, 2048,
+                                       ctypes.byref(p), flags, ctypes.byref(out)))
+            r = _lib.Result(out)
+            n = r.nrows if hasattr(r, "nrows") else None
+            del r
+            return n
+        for _ in range(max(1, args.warmup)):
+            run()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        ctx.synchronize()
+        el = (time.perf_counter() - t0) / args.steps
+        res[name] = nf / el
+        log(f"raw workload, {name}: {nf / el:.0f} frames/s ({el * 1e3:.1f} ms per {nf} frames)")
+    print(json.dumps({"metric": "frames/s, locate + raw signal columns (sig 3, bg 8, gap 1), "
+                                "2048^2 uint16",
+                      "value": res["signal + background"], "unit": "frames/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": 1e3 * nf / res["signal + background"],
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic (seeded device generator), device-resident",
+                      "config": {"workload": f"{nf} x 2048x2048 uint16, raw signal", "params": base},
+                      "locate_only_frames_per_s": res["locate"],
+                      "signal_only_frames_per_s": res["signal"]}), flush=True)
     movie.free()
     return 0
 
