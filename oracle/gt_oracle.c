@@ -37,6 +37,48 @@ int32_t orc_ncols(const orc_params* p) {
     return 4 + (p->cpu_processed ? 3 : 0) + (p->characterize ? 4 : 0);
 }
 
+/* Planted mistake of the characterisation (orc_set_mutation): ORC_MUT_NONE
+ * everywhere but in the tests that show their checks catch each one. */
+static int orc_mut = ORC_MUT_NONE;
+
+int orc_set_mutation(int m) {
+    int old = orc_mut;
+    orc_mut = (m >= 0 && m < ORC_MUT_COUNT) ? m : ORC_MUT_NONE;
+    return old;
+}
+
+/* Disk radius of the characterize columns (centers.wgsl:24: (d - 1) / 2) and
+ * of the cpu_processed columns (execute_gpu.rs:381: circle_inds(d / 2)). */
+static int32_t char_radius_gpu(const orc_geom* g) {
+    return orc_mut == ORC_MUT_GPU_RADIUS_BAND ? g->band : g->rint;
+}
+static int32_t char_radius_cpu(const orc_geom* g) {
+    return orc_mut == ORC_MUT_CPU_RADIUS_RINT ? g->rint : g->band;
+}
+
+int32_t orc_char_tables(int32_t r, float* r2, float* sin2, float* cos2, int32_t cap,
+                        int32_t* mid) {
+    if (r < 0) return -1;
+    int32_t t = 0, m = -1;
+    for (int32_t i = -r; i <= r; i++)
+        for (int32_t j = -r; j <= r; j++) {
+            if (i * i + j * j > r * r) continue;
+            if (t >= cap) return -1;
+            /* kernels.rs:190,201: (2.*(y as f32).atan2(x as f32)).sin() with
+             * x the row offset i and y the column offset j, in binary32 */
+            const float th = atan2f((float)j, (float)i);
+            const float a = (orc_mut == ORC_MUT_THETA) ? th : 2.0f * th;
+            const float q = (float)(i * i + j * j);
+            if (r2) r2[t] = (orc_mut == ORC_MUT_R_NOT_R2) ? sqrtf(q) : q;
+            if (sin2) sin2[t] = sinf(a);
+            if (cos2) cos2[t] = cosf(a);
+            if (i == 0 && j == 0) m = t;
+            t++;
+        }
+    if (mid) *mid = m;
+    return t;
+}
+
 #define REAL float
 #define SFX(name) name##_f32
 #define FMA(a, b, c) fmaf((a), (b), (c))
@@ -126,6 +168,30 @@ int orc_locate_frame(const float* frame, int32_t H, int32_t W,
                         threshold_out);
     free(fd);
     return rc;
+}
+
+int orc_characterise_at(const float* P, const float* raw, int32_t H, int32_t W,
+                        const orc_params* p, int32_t u, int32_t v, float walked_mass,
+                        float* cpu3, float* gpu4) {
+    orc_geom g;
+    if (!P || !raw || orc_geometry(p, &g) != 0) return -1;
+    if (u < 0 || u >= H || v < 0 || v >= W) return -1;
+    const int32_t rc = char_radius_cpu(&g), rg = char_radius_gpu(&g);
+    const int32_t cap_c = (2 * rc + 1) * (2 * rc + 1), cap_g = (2 * rg + 1) * (2 * rg + 1);
+    float* r2t = (float*)malloc(sizeof(float) * (size_t)cap_c * 3);
+    float* tg = (float*)malloc(sizeof(float) * (size_t)cap_g * 2);
+    if (!r2t || !tg) { free(r2t); free(tg); return -2; }
+    float* sint = r2t + cap_c;
+    float* cost = sint + cap_c;
+    int32_t mid = 0;
+    const int32_t ntc = orc_char_tables(rc, r2t, sint, cost, cap_c, &mid);
+    const int32_t ntg = orc_char_tables(rg, NULL, tg, tg + cap_g, cap_g, NULL);
+    if (ntc < 0 || ntg < 0) { free(r2t); free(tg); return -1; }
+    if (cpu3) char_cpu_f32(P, H, W, &g, r2t, sint, cost, ntc, walked_mass, u, v, cpu3);
+    if (gpu4) char_gpu_f32(P, raw, H, W, &g, tg, tg + cap_g, u, v, gpu4);
+    free(r2t);
+    free(tg);
+    return 0;
 }
 
 int orc_candidates_frame(const float* frame, int32_t H, int32_t W,

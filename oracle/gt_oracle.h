@@ -94,6 +94,44 @@ int orc_locate_frame(const float* frame, int32_t H, int32_t W,
 
 int32_t orc_ncols(const orc_params* p);
 
+/* Characterisation tap tables (kernels.rs:171-205 r2/sin/cos_in_circle; the
+ * same sin/cos of 2*atan2(j, i) as centers.wgsl:67-69): one entry per tap of
+ * the disk of integer radius r, row-major over (i, j), every value formed in
+ * binary32 as the reference forms it ((2.*(j as f32).atan2(i as f32)).sin()).
+ * mid: index of the (0, 0) tap (circle_inds' middle_most, kernels.rs:134-147).
+ * Returns the tap count, or -1 when r < 0 or the table exceeds cap. */
+int32_t orc_char_tables(int32_t r, float* r2, float* sin2, float* cos2, int32_t cap,
+                        int32_t* mid);
+
+/* The characterisation columns of one candidate pixel (u, v) of a bandpassed
+ * frame P (raw: the input frame) in binary32:
+ *   cpu3 = Rg, ecc, signal (execute_gpu.rs:165-172,286-299,346-352: the
+ *          neighbourhood over circle_inds(diameter/2) at the candidate,
+ *          divided by the walked mass `walked_mass`);
+ *   gpu4 = Rg_gpu, raw_mass_gpu, signal_gpu, ecc_gpu (centers.wgsl:64-80:
+ *          the disk of radius (diameter-1)/2 at the candidate, divided by
+ *          that disk's own mass; zero in the border band). */
+int orc_characterise_at(const float* P, const float* raw, int32_t H, int32_t W,
+                        const orc_params* p, int32_t u, int32_t v, float walked_mass,
+                        float* cpu3, float* gpu4);
+
+/* Planted mistakes, for the tests that show the characterisation checks
+ * catch them (never set outside those tests).  Returns the previous value. */
+enum {
+    ORC_MUT_NONE = 0,
+    ORC_MUT_GPU_RADIUS_BAND = 1,  /* gpu columns over diameter/2, not (d-1)/2 */
+    ORC_MUT_CPU_RADIUS_RINT = 2,  /* cpu columns over (d-1)/2, not diameter/2 */
+    ORC_MUT_CPU_CAND_MASS = 3,    /* cpu Rg/ecc divided by the candidate's mass */
+    ORC_MUT_THETA = 4,            /* sin/cos of theta instead of 2*theta */
+    ORC_MUT_ECC_FULL_MASS = 5,    /* ecc divided by m, not m - P[centre] */
+    ORC_MUT_SIGNAL_CENTRE = 6,    /* signal = P[centre] instead of the max */
+    ORC_MUT_R_NOT_R2 = 7,         /* Rg weights |r| instead of r^2 */
+    ORC_MUT_MIDDLE = 8,           /* middle_most one tap off */
+    ORC_MUT_RAW_FROM_P = 9,       /* raw_mass_gpu summed over P, not the frame */
+    ORC_MUT_COUNT = 10
+};
+int orc_set_mutation(int m);
+
 /* Pre-dedup candidate records of one frame (after walk and minmass), raster
  * order: cand, y, x, mass.  Used to check deduplication in isolation. */
 int orc_candidates_frame(const float* frame, int32_t H, int32_t W,

@@ -95,6 +95,13 @@ def lib():
         L.orc_link_rows.argtypes = [fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
                                     ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)]
         L.orc_link_rows.restype = ctypes.c_int
+        L.orc_char_tables.argtypes = [ctypes.c_int32, fp, fp, fp, ctypes.c_int32, i32p]
+        L.orc_char_tables.restype = ctypes.c_int32
+        L.orc_characterise_at.argtypes = [fp, fp, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(OrcParams), ctypes.c_int32,
+                                          ctypes.c_int32, ctypes.c_float, fp, fp]
+        L.orc_set_mutation.argtypes = [ctypes.c_int]
+        L.orc_set_mutation.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -308,3 +315,67 @@ def particle_f64(frame: np.ndarray, p: Params, cand: int):
     if rc != 0:
         raise RuntimeError(f"orc_particle_f64 failed ({rc})")
     return yxm, state, int(na.value)
+
+
+# Planted mistakes of the characterisation (gt_oracle.h ORC_MUT_*), for the
+# tests that show their checks catch each one.
+MUTATIONS = {
+    "gpu_radius_band": 1,
+    "cpu_radius_rint": 2,
+    "cpu_candidate_mass": 3,
+    "theta_not_2theta": 4,
+    "ecc_full_mass": 5,
+    "signal_centre": 6,
+    "r_not_r2": 7,
+    "middle_most_off": 8,
+    "raw_from_processed": 9,
+}
+
+
+class mutation:
+    """Context manager: run the oracle with one planted mistake."""
+
+    def __init__(self, name: str):
+        self.m = MUTATIONS[name]
+
+    def __enter__(self):
+        self.old = lib().orc_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(self.old)
+        return False
+
+
+def char_tables(r: int):
+    """(r2, sin2, cos2, middle index) of the disk of integer radius r, binary32."""
+    cap = (2 * r + 1) ** 2
+    r2 = np.zeros(cap, np.float32)
+    s2 = np.zeros(cap, np.float32)
+    c2 = np.zeros(cap, np.float32)
+    mid = ctypes.c_int32()
+    n = lib().orc_char_tables(r, _ptr(r2, ctypes.c_float), _ptr(s2, ctypes.c_float),
+                              _ptr(c2, ctypes.c_float), cap, ctypes.byref(mid))
+    if n < 0:
+        raise ValueError("invalid radius")
+    return r2[:n].copy(), s2[:n].copy(), c2[:n].copy(), int(mid.value)
+
+
+def characterise_at(P: np.ndarray, raw: np.ndarray, p: Params, u: int, v: int,
+                    walked_mass: float):
+    """(Rg, ecc, signal) and (Rg_gpu, raw_mass_gpu, signal_gpu, ecc_gpu) of the
+    candidate pixel (u, v) of the bandpassed frame P, binary32."""
+    Pc = np.ascontiguousarray(P, np.float32)
+    Rc = np.ascontiguousarray(raw, np.float32)
+    if Pc.shape != Rc.shape:
+        raise ValueError("P and raw differ in shape")
+    H, W = Pc.shape
+    cpu3 = np.zeros(3, np.float32)
+    gpu4 = np.zeros(4, np.float32)
+    cp = p.c()
+    rc = lib().orc_characterise_at(_ptr(Pc, ctypes.c_float), _ptr(Rc, ctypes.c_float), H, W,
+                                   ctypes.byref(cp), int(u), int(v), float(walked_mass),
+                                   _ptr(cpu3, ctypes.c_float), _ptr(gpu4, ctypes.c_float))
+    if rc != 0:
+        raise ValueError(f"orc_characterise_at rejected input (rc={rc})")
+    return cpu3, gpu4

@@ -293,15 +293,17 @@ gt_status make_geom(const gt_params* p, bool integer_input, Geom& g) {
     g.dedup_rty = (int32_t)(D / TILE_H) + 1;
     g.dedup_rtx = (int32_t)(D / TILE_W) + 1;
     g.ncols = ncols_of(p);
-    // characterise tables (centers.wgsl:67-69 sin/cos of 2*atan2(j, i))
+    // characterise tables (centers.wgsl:67-69 sin/cos of 2*atan2(j, i)), in
+    // binary32 as kernels.rs:185-205 forms them: (2.*(j as f32).atan2(i as
+    // f32)).sin() with the row offset i and the column offset j
     int t = 0;
     for (int i = -g.rint; i <= g.rint; ++i)
         for (int j = -g.rint; j <= g.rint; ++j) {
             if (i * i + j * j > g.rint * g.rint) continue;
             if (t >= 1024) return fail(GT_E_INVALID, "diameter too large for characterisation");
-            const double th = std::atan2((double)j, (double)i);
-            g.sin2_gpu[t] = (float)std::sin(2.0 * th);
-            g.cos2_gpu[t] = (float)std::cos(2.0 * th);
+            const float a = 2.0f * atan2f((float)j, (float)i);
+            g.sin2_gpu[t] = sinf(a);
+            g.cos2_gpu[t] = cosf(a);
             ++t;
         }
     g.ntap_gpu = t;
@@ -311,10 +313,10 @@ gt_status make_geom(const gt_params* p, bool integer_input, Geom& g) {
         for (int j = -g.band; j <= g.band; ++j) {
             if (i * i + j * j > g.band * g.band) continue;
             if (t >= 1024) return fail(GT_E_INVALID, "diameter too large for characterisation");
-            const double th = std::atan2((double)j, (double)i);
+            const float a = 2.0f * atan2f((float)j, (float)i);
             g.r2_cpu[t] = (float)(i * i + j * j);
-            g.sin_cpu[t] = (float)std::sin(2.0 * th);
-            g.cos_cpu[t] = (float)std::cos(2.0 * th);
+            g.sin_cpu[t] = sinf(a);
+            g.cos_cpu[t] = cosf(a);
             ++t;
         }
     g.ntap_cpu = t;
