@@ -175,3 +175,49 @@ def test_chunked_linking_matches_oracle(ctx):
     assert np.array_equal(host_rows[:, :4], plain_rows[:, :4])
     ref = _oracle_per_frame(plain_rows, 600, 4.0, 2)
     assert np.array_equal(host_rows[:, 4], ref.astype(np.float32))
+
+
+def test_full_size_c5_locate_and_link(ctx):
+    """The locate + link workload at full size: 60 frames of 1024x1024 with
+    3000 Brownian particles (D = 0.5 px^2/frame, sigma 1.8, 5 % blinking),
+    device-resident; diameter 9, separation 9, minmass from frame 0 as the
+    benchmark sets it, search_range 5, memory 3.  Rows of three frames equal
+    the oracle's locate; the particle ids of every row equal the C oracle
+    linker's (oracle/gt_linker.c) on the located rows."""
+    import ctypes
+    from oracle import oracle
+    nf, H, W = 60, 1024, 1024
+    spec = synth.link_1024(seed=5, n_frames=nf)
+    movie = synth.generate(ctx, spec)
+    L = _lib.load()
+
+    def locate_rows(p, n):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), n, H, W,
+                                   ctypes.byref(p), _lib.GT_FRAMES_ON_DEVICE, ctypes.byref(out)))
+        return _lib.Result(out)
+
+    kw = dict(diameter=9, separation=9)
+    r0 = locate_rows(make_params(**kw, minmass=0.0), 1).rows()
+    top = np.sort(r0[:, 1])[::-1][:int(spec.spot_off[1])]
+    kw["minmass"] = float(0.25 * np.median(top))
+    print(f"[link] full-size C5: minmass {kw['minmass']:.1f}", flush=True)
+    res = locate_rows(make_params(**kw, search_range=5.0, memory=3), nf)
+    rows, cand = res.rows(), res.candidates()
+    plain = locate_rows(make_params(**kw), nf).rows()
+    assert rows.shape == (plain.shape[0], plain.shape[1] + 1)
+    assert np.array_equal(rows[:, :4], plain[:, :4])
+    assert len(np.unique(plain[:, 0])) == nf          # no empty frame: steps = frames
+    assert plain.shape[0] > nf * 2500
+    host = movie.to_host(np.uint16, (nf, H, W))
+    movie.free()
+    op = oracle.Params(diameter=9, separation=9, minmass=kw["minmass"])
+    for f in (0, nf // 2, nf - 1):
+        ref_rows, ref_cand, _ = oracle.locate_frame(host[f].astype(np.float32), op, frame_index=f)
+        sel = rows[:, 0] == f
+        assert np.array_equal(cand[sel], ref_cand), f
+        assert np.abs(rows[sel][:, 2:4] - ref_rows[:, 2:4]).max() <= 1e-4
+        assert (np.abs(rows[sel][:, 1] - ref_rows[:, 1]) / ref_rows[:, 1]).max() <= 1e-5
+    ref_ids = oracle.link_rows_c(plain, 5.0, 3)
+    print(f"[link] full-size C5: {rows.shape[0]} rows, {int(ref_ids.max()) + 1} ids", flush=True)
+    assert np.array_equal(rows[:, 4], ref_ids.astype(np.float32))

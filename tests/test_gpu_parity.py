@@ -326,3 +326,114 @@ def test_host_result_blocks_reused(ctx):
     _check(rows, cand, ref_rows, ref_cand, "reused host blocks")
     rows2, cand2, _, _ = locate(host, make_params(**kw), ctx=ctx)
     assert np.array_equal(rows, rows2) and np.array_equal(cand, cand2)
+
+
+def _max_staged_records(frame, kw, tile=64):
+    """Largest number of pre-dedup records any tile's dedup has to stage: the
+    valid records of the tiles within its window (separation + 2 (max_iter +
+    rint) + 1 px) that lie within separation + 1 of the bounding box of the
+    tile's own records.  Computed on the CPU from the oracle's candidates."""
+    p = _oracle_params(kw)
+    cand, y, x, m = oracle.candidates_frame(frame.astype(np.float32), p)
+    H, W = frame.shape
+    g = oracle.geometry(p)
+    sep = p.c().separation
+    D = np.ceil(sep) + 2.0 * (p.max_iterations + g.rint) + 1.0
+    rt = int(D // tile) + 1
+    ty, tx = (cand // W) // tile, (cand % W) // tile
+    best, ntl_max = 0, 0
+    for a in np.unique(ty):
+        for b in np.unique(tx[ty == a]):
+            own = (ty == a) & (tx == b)
+            win = (np.abs(ty - a) <= rt) & (np.abs(tx - b) <= rt)
+            ntl_max = max(ntl_max, (min(a + rt, (H - 1) // tile) - max(a - rt, 0) + 1) *
+                          (min(b + rt, (W - 1) // tile) - max(b - rt, 0) + 1))
+            mg = sep + 1.0
+            box = ((y >= y[own].min() - mg) & (y <= y[own].max() + mg) &
+                   (x >= x[own].min() - mg) & (x <= x[own].max() + mg))
+            best = max(best, int((win & box).sum()))
+    return best, ntl_max
+
+
+def test_dense_dedup_stages_in_rounds(ctx):
+    """A noisy frame with a small disk and separation gives hundreds of
+    records per dedup neighbourhood, more than one staging round holds (256):
+    rows still equal the oracle."""
+    rng = np.random.default_rng(21)
+    frame = rng.poisson(300, (1, 384, 320)).astype(np.uint16)
+    for kw in (dict(diameter=3, separation=3, smoothing_size=3),      # two rounds
+               dict(diameter=3, separation=2, smoothing_size=5)):     # seven rounds
+        staged, _ = _max_staged_records(frame[0], kw)
+        print(f"[parity] dense dedup {kw}: up to {staged} staged records per tile", flush=True)
+        assert staged > 256, staged
+        ref_rows, ref_cand = _oracle_movie(frame, kw)
+        for plain in (False, True):
+            rows, cand, _, _ = locate(frame, make_params(**kw), plain=plain, ctx=ctx)
+            _check(rows, cand, ref_rows, ref_cand, f"dense dedup {kw} plain={plain}")
+
+
+@pytest.mark.parametrize("kw", [dict(diameter=9, separation=9, max_iterations=100),
+                                dict(diameter=9, separation=200, percentile=64)],
+                         ids=["max_iterations_100", "separation_200"])
+def test_wide_dedup_window(ctx, kw):
+    """Dedup windows wider than 64 tiles (separation + 2 (max_iterations +
+    rint) + 1 > 192 px) on 1024x1024 frames: the neighbour tiles are taken in
+    groups of 64, and rows equal the oracle."""
+    spec = synth.link_1024(seed=5, n_frames=2)
+    frames = synth.generate_host(ctx, spec)
+    _, ntl = _max_staged_records(frames[0], kw)
+    print(f"[parity] wide dedup window {kw}: {ntl} neighbour tiles", flush=True)
+    assert ntl > 64, ntl
+    rows, cand, _, _ = locate(frames, make_params(**kw), ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(frames, kw)
+    _check(rows, cand, ref_rows, ref_cand, f"wide window {kw}")
+
+
+def test_uneven_workload_content(ctx):
+    """Frames of the uneven benchmark batch itself (its generator: one spot per
+    800 px^2, sigma U(1.2, 2.5), lognormal peaks, background U(50, 300) per
+    frame; 256x256, 512x1024 and 2048x2048 shuffled), device-resident in one
+    gt_locate_batch call, batched and plain: every frame equals the oracle."""
+    import ctypes
+    from tracking_optimizations_amd._lib import Result, check, load
+    nf = 20
+    movie = synth.UnevenMovie(ctx, nf, seed=3)
+    descs = movie.frame_descs()
+    kw = CASES["c2"]
+    p = make_params(**kw)
+    host = []
+    for (ptr, h, w) in movie.descs:
+        f = np.empty((h, w), np.uint16)
+        check(load().gt_memcpy_d2h(ctx.handle, f.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.c_void_p(ptr), f.nbytes))
+        host.append(f)
+    assert {f.shape for f in host} == {(256, 256), (512, 1024), (2048, 2048)}
+    refs = [oracle.locate_frame(f.astype(np.float32), _oracle_params(kw), frame_index=i)
+            for i, f in enumerate(host)]
+    for flags in (_lib.GT_FRAMES_ON_DEVICE, _lib.GT_FRAMES_ON_DEVICE | _lib.GT_PLAIN):
+        out = ctypes.c_void_p()
+        print(f"[parity] uneven workload content flags {flags}", flush=True)
+        check(load().gt_locate_batch(ctx.handle, descs, nf, 0, ctypes.byref(p), flags,
+                                     ctypes.byref(out)))
+        r = Result(out)
+        rows, cand = r.rows(), r.candidates()
+        for i, (ref_rows, ref_cand, _) in enumerate(refs):
+            sel = rows[:, 0] == i
+            _check(rows[sel], cand[sel], ref_rows, ref_cand, f"uneven content {i} {host[i].shape}")
+    movie.free()
+
+
+def test_graph_replay_host_chunks(ctx):
+    """Host frames of 40 x 2048^2 pass through the two pinned chunks (31
+    frames each), so consecutive chunks differ in their staging buffer: with
+    graphs on, each chunk's captured sequence is cached and replayed on the
+    next call, and every replay gives the rows of direct launches."""
+    host = synth.generate_host(ctx, synth.movie_2048(seed=2, n_frames=40))
+    p = make_params(**CASES["c2"])
+    ref = locate(host, p, ctx=_lib.Context(0))
+    g = _lib.Context(0)
+    g.set_graphs(True)
+    for rep in range(3):
+        print(f"[parity] graph replay of host chunks {rep}", flush=True)
+        got = locate(host, p, ctx=g)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), rep

@@ -363,9 +363,20 @@ struct gt_context {
     // batched launch sequence captured as a graph and replayed while the
     // call's inputs, parameters and buffers are unchanged
     int graphs = 0;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<uint64_t> gkey;
-    uint64_t glaunches = 0;
+    // (call key, executable graph, launches) of recent calls, most recent
+    // last: host-staged calls alternate their chunks between two pinned
+    // buffers, so consecutive chunks have different keys
+    struct GraphEntry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec;
+        uint64_t launches;
+    };
+    static constexpr size_t kGraphCache = 4;
+    std::vector<GraphEntry> gcache;
+    void drop_graphs() {
+        for (auto& e : gcache) (void)hipGraphExecDestroy(e.exec);
+        gcache.clear();
+    }
     // autotuned (rows per bandpass work item, frames per launch) per frame
     // shape and filter geometry
     std::map<std::array<int64_t, 6>, std::pair<int, uint32_t>> tuned;
@@ -793,7 +804,8 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         generic = !bandpass_fast_path(probe, g);
     }
 
-    gt_result* res = new gt_result();
+    // owned here until it is handed to the caller: every error return frees it
+    std::unique_ptr<gt_result> res(new gt_result());
     res->ncols = (uint32_t)g.ncols;
     {
         const char* names[16];
@@ -808,7 +820,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     ctx->last = gt_timing{};
     ctx->last.frames = frames.size();
     if (frames.empty()) {
-        *out = res;
+        *out = res.release();
         return GT_OK;
     }
 
@@ -1099,9 +1111,9 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         return GT_OK;
         };  // enqueue
 
-        // Replay the captured launch sequence when this call repeats the
-        // previous one (same inputs, parameters, launch shape and buffers);
-        // capture it otherwise.  Any capture failure falls back to launching
+        // Replay a captured launch sequence when this call repeats one of the
+        // last kGraphCache calls (same inputs, parameters, launch shape and
+        // buffers); capture it otherwise.  Any capture failure falls back to launching
         // directly and turns graphs off for the context.
         bool launched = false;
         if (ctx->graphs && !plain && !ctx->profiling && !adj) {
@@ -1126,32 +1138,40 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                                       &w.rec, &w.keep, &w.tile3, &w.perframe};
                 for (const DevBuf* d : wb) key.push_back((uint64_t)(uintptr_t)d->p);
             }
-            if (!(ctx->gexec && ctx->gkey == key)) {
-                if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-                ctx->gexec = nullptr;
-                ctx->gkey.clear();
+            size_t hit = ctx->gcache.size();
+            for (size_t i = 0; i < ctx->gcache.size(); ++i)
+                if (ctx->gcache[i].key == key) hit = i;
+            if (hit == ctx->gcache.size()) {
+                hipGraphExec_t ex = nullptr;
                 if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) == hipSuccess) {
                     const gt_status st = enqueue();
                     hipGraph_t gr = nullptr;
                     const hipError_t e = hipStreamEndCapture(s0, &gr);
-                    hipGraphExec_t ex = nullptr;
-                    if (st == GT_OK && e == hipSuccess && gr &&
-                        hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess) {
-                        ctx->gexec = ex;
-                        ctx->gkey = key;
-                        ctx->glaunches = launches;
-                    }
+                    if (!(st == GT_OK && e == hipSuccess && gr &&
+                          hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess))
+                        ex = nullptr;
                     if (gr) (void)hipGraphDestroy(gr);
                 }
-                if (!ctx->gexec) {
+                if (ex) {
+                    if (ctx->gcache.size() == gt_context::kGraphCache) {
+                        (void)hipGraphExecDestroy(ctx->gcache.front().exec);
+                        ctx->gcache.erase(ctx->gcache.begin());
+                    }
+                    ctx->gcache.push_back({key, ex, launches});
+                    hit = ctx->gcache.size() - 1;
+                } else {
                     ctx->graphs = 0;
                     (void)hipGetLastError();
                 }
             }
-            if (ctx->gexec) {
+            if (hit < ctx->gcache.size()) {
+                // most recent last
+                std::rotate(ctx->gcache.begin() + (std::ptrdiff_t)hit,
+                            ctx->gcache.begin() + (std::ptrdiff_t)hit + 1, ctx->gcache.end());
+                const auto& e = ctx->gcache.back();
                 HIP_TRY(hipEventRecord(ctx->ev0, s0));
-                HIP_TRY(hipGraphLaunch(ctx->gexec, s0));
-                launches = ctx->glaunches;
+                HIP_TRY(hipGraphLaunch(e.exec, s0));
+                launches = e.launches;
                 launched = true;
             }
         }
@@ -1189,18 +1209,12 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         }
         break;
     }
-    if (p->has_search_range) {
-        const gt_status ls = link_result(ctx, frames, p, res);
-        if (ls != GT_OK) {
-            delete res;
-            return ls;
-        }
-    }
+    if (p->has_search_range) GT_TRY(link_result(ctx, frames, p, res.get()));
     ctx->last.rows = res->rows;
     ctx->last.launches = launches;
     ctx->last.total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    *out = res;
+    *out = res.release();
     return GT_OK;
 }
 
@@ -1340,7 +1354,7 @@ gt_status gt_context_destroy(gt_context* ctx) {
     if (!ctx) return GT_OK;
     (void)hipSetDevice(ctx->device);
     for (int i = 0; i < kStreams; ++i) (void)hipStreamSynchronize(ctx->streams[i]);
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+    ctx->drop_graphs();
     DevBuf* bufs[] = {&ctx->geom, &ctx->frames_tab, &ctx->bases, &ctx->rows, &ctx->rows_cand,
                       &ctx->counters};
     for (DevBuf* b : bufs) b->release();
@@ -1399,11 +1413,9 @@ gt_status gt_context_set_launch_shape(gt_context* ctx, uint32_t seg_rows) {
 gt_status gt_context_set_graphs(gt_context* ctx, int enable) {
     if (!ctx) return fail(GT_E_INVALID, "context is NULL");
     ctx->graphs = enable ? 1 : 0;
-    if (!enable && ctx->gexec) {
+    if (!enable && !ctx->gcache.empty()) {
         (void)hipSetDevice(ctx->device);
-        (void)hipGraphExecDestroy(ctx->gexec);
-        ctx->gexec = nullptr;
-        ctx->gkey.clear();
+        ctx->drop_graphs();
     }
     return GT_OK;
 }

@@ -1715,14 +1715,14 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 // ---------------------------------------------------------------------------
 // K4 dedup: one wave per tile.  The valid records (mass != 0) of the tiles
 // that can hold a neighbour, and that lie within `separation` of the
-// bounding box of the tile's own valid positions, are staged in LDS, one per
-// lane.  Each own candidate is then broadcast to the wave and every lane
-// tests one staged record against it (execute_gpu.rs:258-284: drop when a
-// record strictly within `separation` has a larger mass, or an equal mass and
-// a lower raster index).  Staging areas that would overflow fall back to
-// scanning the records in global memory.
+// bounding box of the tile's own valid positions, are staged in LDS as
+// 16-byte entries.  Each lane then tests its own candidate against every
+// staged record (execute_gpu.rs:258-284: drop when a record strictly within
+// `separation` has a larger mass, or an equal mass and a lower raster
+// index).  Neighbourhoods larger than the staging area are staged and tested
+// in rounds.
 // ---------------------------------------------------------------------------
-constexpr int DD_CAP = 256;   // staged records per wave (more fall back to global scans)
+constexpr int DD_CAP = 256;   // staged records per wave and round
 
 __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__ g) {
     // staged records, one 16-byte entry each: mass, y, x, raster index (bits)
@@ -1786,55 +1786,20 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     const float margin = rad + 1.0f;
     bymin -= margin; bymax += margin; bxmin -= margin; bxmax += margin;
 
-    // stage the neighbourhood: the counts of all neighbour tiles are read in
-    // one round (lane q holds tile q), then the records of all tiles are
-    // enumerated 64 at a time, so the loads of a round are independent
+    // The neighbourhood is staged in rounds of at most DD_CAP records: the
+    // neighbour tiles are taken 64 at a time (lane q holds tile q's count),
+    // their records enumerated 64 at a time (the loads of a round are
+    // independent) and filtered by the bounding box; when the next 64 could
+    // overflow the staging area, the own candidates are tested against what
+    // is staged and staging resumes.  When the whole neighbourhood fits in
+    // one round it is staged once and serves every batch of own candidates.
     uint32_t S = 0;
-    bool staged = ntl <= 64;
-    if (staged) {
-        uint32_t myc = 0, myit = 0;
-        if (lane < ntl) {
-            const int ny = y_lo + lane / nxr, nx = x_lo + lane % nxr;
-            myit = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-            myc = b.tile_count[myit];
-        }
-        const uint32_t inc = wave_incl_scan(myc, lane);
-        const uint32_t exq = inc - myc;
-        const uint32_t T = __shfl(inc, 63, 64);
-        for (uint32_t jb = 0; jb < T && staged; jb += 64) {
-            const uint32_t j = jb + lane;
-            // tile of record j: the last q whose first record is <= j
-            int q = 0;
-            for (int qq = 1; qq < ntl; ++qq)  // qq is wave-uniform: a scalar read of lane qq
-                if ((uint32_t)__builtin_amdgcn_readlane((int)exq, qq) <= j) q = qq;
-            const uint32_t it2 = __shfl(myit, q, 64);
-            const uint32_t i2 = j - __shfl(exq, q, 64);
-            const size_t s2 = (size_t)it2 * TILE_PX + i2;
-            bool take = false;
-            float m = 0.f, y = 0.f, x = 0.f;
-            if (j < T) {
-                m = rmass[s2];
-                y = ry[s2];
-                x = rx[s2];
-                take = ok(m, s2) && y >= bymin && y <= bymax && x >= bxmin && x <= bxmax;
-            }
-            const uint64_t tm = __ballot(take);
-            const uint32_t nt = (uint32_t)__popcll(tm);
-            if (S + nt > (uint32_t)DD_CAP) { staged = false; break; }
-            if (take) {
-                const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
-                const uint32_t l2 = slots[s2];
-                const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
-                const int32_t ri = (ny * TILE_H + (int)(l2 / TILE_W)) * W + (nx * TILE_W + (int)(l2 % TILE_W));
-                s_rec[wave][r] = make_float4(m, y, x, __int_as_float(ri));
-            }
-            S += nt;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
+    bool resident = false;
+    auto wave_sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (uint32_t ib = 0; ib < cnt; ib += 64) {
         const uint32_t i = ib + lane;
         const bool own = i < cnt;
@@ -1848,50 +1813,83 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
             idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
         }
         const bool valid = own && ok(m, slot0 + i);
-        bool keep = valid;
-        if (staged) {
-            // every lane tests its own candidate against the staged records
-            // in turn (one broadcast 16-byte LDS read per record; the reads
-            // do not depend on each other, so they pipeline)
-            bool drop = false;
-            if (__ballot(valid)) {  // wave-uniform
+        bool drop = false;
+        // every lane tests its own candidate against the staged records in
+        // turn (one broadcast 16-byte LDS read per record; the reads do not
+        // depend on each other, so they pipeline)
+        auto test_staged = [&](uint32_t n) {
 #pragma unroll 4
-                for (uint32_t j = 0; j < S; ++j) {
-                    const float4 rr = s_rec[wave][j];
-                    const float dy = rr.y - y, dx = rr.z - x;
-                    float d = dy * dy;
-                    d = d + dx * dx;
-                    const int32_t nidx = __float_as_int(rr.w);
-                    drop |= d < r2 && nidx != idx && (m < rr.x || (m == rr.x && idx > nidx));
-                }
+            for (uint32_t j = 0; j < n; ++j) {
+                const float4 rr = s_rec[wave][j];
+                const float dy = rr.y - y, dx = rr.z - x;
+                float d = dy * dy;
+                d = d + dx * dx;
+                const int32_t nidx = __float_as_int(rr.w);
+                drop |= d < r2 && nidx != idx && (m < rr.x || (m == rr.x && idx > nidx));
             }
-            keep = valid && !drop;
-        } else if (__ballot(keep)) {
-            for (int ny = y_lo; ny <= y_hi; ++ny) {
-                for (int nx = x_lo; nx <= x_hi; ++nx) {
-                    const uint32_t t2 = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
-                    const uint32_t c2 = b.tile_count[t2];
-                    const size_t s2 = (size_t)t2 * TILE_PX;
-                    for (uint32_t j = 0; j < c2; ++j) {
-                        const float nm = rmass[s2 + j];
-                        if (!ok(nm, s2 + j)) continue;
-                        const float dy = ry[s2 + j] - y, dx = rx[s2 + j] - x;
-                        float d = dy * dy;
-                        d = d + dx * dx;
-                        if (d < r2 && !(t2 == item && j == i)) {
-                            bool drop = m < nm;
-                            if (!drop && m == nm) {
-                                const uint32_t l2 = slots[s2 + j];
-                                const int32_t nidx = (ny * TILE_H + (int)(l2 / TILE_W)) * W +
-                                                     (nx * TILE_W + (int)(l2 % TILE_W));
-                                drop = idx > nidx;
-                            }
-                            if (drop) keep = false;
-                        }
+        };
+        if (__ballot(valid)) {  // wave-uniform
+            if (resident) {
+                test_staged(S);
+            } else {
+                bool single = true;   // one round held the whole neighbourhood
+                for (int g0 = 0; g0 < ntl; g0 += 64) {
+                    const int ng = min(64, ntl - g0);
+                    uint32_t myc = 0, myit = 0;
+                    if (lane < ng) {
+                        const int q = g0 + lane;
+                        const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
+                        myit = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
+                        myc = b.tile_count[myit];
                     }
+                    const uint32_t inc = wave_incl_scan(myc, lane);
+                    const uint32_t exq = inc - myc;
+                    const uint32_t T = __shfl(inc, 63, 64);
+                    uint32_t jb = 0;
+                    do {
+                        S = 0;
+                        for (; jb < T; jb += 64) {
+                            const uint32_t j = jb + lane;
+                            // tile of record j: the last q whose first record is <= j
+                            int q = 0;
+                            for (int qq = 1; qq < ng; ++qq)  // wave-uniform: scalar read of lane qq
+                                if ((uint32_t)__builtin_amdgcn_readlane((int)exq, qq) <= j) q = qq;
+                            const uint32_t it2 = __shfl(myit, q, 64);
+                            const uint32_t i2 = j - __shfl(exq, q, 64);
+                            const size_t s2 = (size_t)it2 * TILE_PX + i2;
+                            bool take = false;
+                            float nm = 0.f, yy = 0.f, xx = 0.f;
+                            if (j < T) {
+                                nm = rmass[s2];
+                                yy = ry[s2];
+                                xx = rx[s2];
+                                take = ok(nm, s2) && yy >= bymin && yy <= bymax && xx >= bxmin &&
+                                       xx <= bxmax;
+                            }
+                            const uint64_t tm = __ballot(take);
+                            const uint32_t nt = (uint32_t)__popcll(tm);
+                            if (S + nt > (uint32_t)DD_CAP) break;   // test, then resume at jb
+                            if (take) {
+                                const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
+                                const uint32_t l2 = slots[s2];
+                                const int qg = g0 + q;
+                                const int ny = y_lo + qg / nxr, nx = x_lo + qg % nxr;
+                                const int32_t ri = (ny * TILE_H + (int)(l2 / TILE_W)) * W +
+                                                   (nx * TILE_W + (int)(l2 % TILE_W));
+                                s_rec[wave][r] = make_float4(nm, yy, xx, __int_as_float(ri));
+                            }
+                            S += nt;
+                        }
+                        wave_sync();
+                        test_staged(S);
+                        if (jb < T || g0 + 64 < ntl) single = false;
+                        wave_sync();   // every lane's reads done before the next round's writes
+                    } while (jb < T);
                 }
+                resident = single;
             }
         }
+        const bool keep = valid && !drop;
         if (own) b.keep[slot0 + i] = keep ? 1 : 0;
         if (keep) atomicAdd(&s_kr[wave][(uint32_t)(idx / W) - (uint32_t)(ty * TILE_H)], 1u);
         kept += (uint32_t)__popcll(__ballot(keep));
