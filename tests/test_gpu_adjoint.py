@@ -111,32 +111,40 @@ def test_torch_autograd_matches_vjp_and_directional_derivative(ctx, frames):
     _, _, _, a = locate_errors(frames.astype(np.float32), p, ctx=ctx, jacobian=True)
     g = np.tile(np.array([[1.0, -1.0, 1e-3]], np.float32), (a.nrows, 1))  # (y, x, mass)
     assert np.array_equal(grad, a.vjp(ctx, g))
-    # directional derivative: a small push d; rows whose candidate pixel is
-    # found in all three runs and whose walk ends on the same pixel (the
-    # position moves by well under half a pixel) are compared, the others
-    # excluded from both sides; the check must run on most rows
+    # directional derivative: a small push d.  Rows whose candidate pixel is
+    # found in all three runs and whose walk evaluates its last disk at the
+    # same pixel (same Jacobian patch origin) lie on one smooth piece and are
+    # compared; the others (a walk decision or a candidate that flips) are
+    # excluded from both sides.  The check must run on most rows.
     rng = np.random.default_rng(2)
     d = rng.normal(size=frames.shape).astype(np.float32) * 0.25
     base = frames.astype(np.float32)
-    r0, c0, _, _ = locate(base, p, ctx=ctx)
-    rp, cp, _, _ = locate(base + d, p, ctx=ctx)
-    rm, cm, _, _ = locate(base - d, p, ctx=ctx)
+    r0, c0, a0 = rows_origin(base, p, ctx)
+    rp, cp, ap = rows_origin(base + d, p, ctx)
+    rm, cm, am = rows_origin(base - d, p, ctx)
+    assert np.array_equal(r0, a.rows())
     key = lambda r, c: r[:, 0].astype(np.int64) * (1 << 40) + c
     k0, kp, km = key(r0, c0), key(rp, cp), key(rm, cm)
+    assert np.all(np.diff(k0) > 0) and np.all(np.diff(kp) > 0) and np.all(np.diff(km) > 0)
     common = np.intersect1d(np.intersect1d(k0, kp), km)
     i0 = np.searchsorted(k0, common)
     ip = np.searchsorted(kp, common)
     im = np.searchsorted(km, common)
-    assert np.all(np.diff(k0) > 0) and np.all(np.diff(kp) > 0) and np.all(np.diff(km) > 0)
-    same_walk = (np.abs(rp[ip, 2:4] - rm[im, 2:4]).max(axis=1) < 0.3)
-    i0, ip, im = i0[same_walk], ip[same_walk], im[same_walk]
-    assert a.nrows == len(k0)
+    same = np.all(a0[i0] == ap[ip], axis=1) & np.all(a0[i0] == am[im], axis=1)
+    i0, ip, im = i0[same], ip[same], im[same]
     assert len(i0) >= 0.9 * len(k0), (len(i0), len(k0))
-    f = lambda r, idx: float((r[idx, 2] - r[idx, 3]).sum() + 1e-3 * r[idx, 1].sum())
+    f = lambda r, idx: float((r[idx, 2].astype(np.float64) - r[idx, 3]).sum() +
+                             1e-3 * r[idx, 1].astype(np.float64).sum())
     fd = (f(rp, ip) - f(rm, im)) / 2
     gsel = np.zeros_like(g)
     gsel[i0] = g[i0]
-    lin = float((a.vjp(ctx, gsel) * d).sum())
+    lin = float((a.vjp(ctx, gsel).astype(np.float64) * d).sum())
     print(f"[adjoint] directional derivative on {len(i0)} of {len(k0)} rows: fd {fd:.5g} "
           f"linear {lin:.5g}", flush=True)
     assert abs(fd - lin) <= 0.05 * abs(lin) + 1e-2
+
+
+def rows_origin(video, p, ctx):
+    """Rows, candidates and the last evaluated disk's patch origin per row."""
+    rows, cand, _, a = locate_errors(video, p, ctx=ctx)
+    return rows, cand, a.origin().copy()

@@ -208,7 +208,7 @@ def test_full_size_c5_locate_and_link(ctx):
     assert rows.shape == (plain.shape[0], plain.shape[1] + 1)
     assert np.array_equal(rows[:, :4], plain[:, :4])
     assert len(np.unique(plain[:, 0])) == nf          # no empty frame: steps = frames
-    assert plain.shape[0] > nf * 2500
+    assert plain.shape[0] > nf * 1500
     host = movie.to_host(np.uint16, (nf, H, W))
     movie.free()
     op = oracle.Params(diameter=9, separation=9, minmass=kw["minmass"])

@@ -50,6 +50,9 @@ CASES = {
     # dilation halo (3) narrower than the disk radius (5): walks near tile
     # edges read their disks from P instead of the staged tile
     "halo_fallback": dict(diameter=11, separation=5, smoothing_size=9, percentile=50),
+    # dilation window of 70 px: too wide for a staged tile, the row maxima go
+    # through a plane
+    "sep_50": dict(diameter=9, separation=50, percentile=64),
 }
 
 

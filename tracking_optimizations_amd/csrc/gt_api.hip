@@ -546,10 +546,10 @@ struct Need {
     size_t nframes = 0;
 };
 
-size_t workspace_bytes(const Need& n, const Geom& g, bool generic) {
+size_t workspace_bytes(const Need& n, const Geom& g, bool tmp_planes) {
     size_t b = 0;
     b += al256(n.px * 4);                                   // P
-    if (generic) b += al256(n.px * 8);                      // tmp
+    if (tmp_planes) b += al256(n.px * 8);                   // tmp
     if (g.has_pct) b += al256(n.tiles * TILE_PX * 4);       // list
     b += al256(n.tiles * TILE_PX * 2);                      // slots
     b += al256(n.tiles * TILE_PX * 4 * (uint64_t)rec_planes(g));  // rec
@@ -803,6 +803,9 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         probe.vec = vec ? 1 : 0;
         generic = !bandpass_fast_path(probe, g);
     }
+    // per-frame tmp planes: the generic bandpass's row pass, or the row
+    // maxima of a dilation window too wide for detect's staged tile
+    const bool tmp_planes = generic || detect_needs_plane(g);
 
     // owned here until it is handed to the caller: every error return frees it
     std::unique_ptr<gt_result> res(new gt_result());
@@ -841,7 +844,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                 m.k1 += fs.k1_items;
                 m.in_bytes += on_device ? 0 : al256(fs.px * es);
                 m.nframes += 1;
-                if (b1 > b0 && workspace_bytes(m, g, generic) > ctx->workspace_limit) break;
+                if (b1 > b0 && workspace_bytes(m, g, tmp_planes) > ctx->workspace_limit) break;
                 n = m;
                 ++b1;
             }
@@ -871,7 +874,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     for (int si = 0; si < nstreams; ++si) {
         Workspace& w = ctx->ws[si];
         HIP_TRY(w.P.ensure(maxneed.px * 4));
-        if (generic) HIP_TRY(w.tmp.ensure(maxneed.px * 8));
+        if (tmp_planes) HIP_TRY(w.tmp.ensure(maxneed.px * 8));
         if (g.has_pct) HIP_TRY(w.list.ensure(slot_cap * 4));
         HIP_TRY(w.slots.ensure(slot_cap * 2));
         HIP_TRY(w.rec.ensure(slot_cap * 4 * (uint64_t)rec_planes(g)));
@@ -911,7 +914,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                 inoff += al256(fs.px * es);
             }
             F.P = wsb.P.as<float>() + pxo;
-            F.tmp = generic ? wsb.tmp.as<float>() + 2 * pxo : nullptr;
+            F.tmp = tmp_planes ? wsb.tmp.as<float>() + 2 * pxo : nullptr;
             F.H = frames[fi].h;
             F.W = frames[fi].w;
             F.k1_tx = fs.k1_tx;

@@ -44,7 +44,8 @@ enum RecPlane : int {
 struct Frame {
     const void* in;        // H*W input pixels (device)
     float* P;              // H*W bandpassed frame
-    float* tmp;            // 2*H*W row-pass temporary (generic bandpass only)
+    float* tmp;            // 2*H*W row-pass temporary (generic bandpass), then the
+                           // row-max plane of wide dilation windows
     uint64_t slot_base;    // first candidate slot of this frame
     uint32_t H, W;
     uint32_t k1_tx, k1_ty; // bandpass strips x segments
@@ -147,6 +148,9 @@ hipError_t launch_bandpass_generic(const Batch& b, const Geom* d_geom, uint32_t 
                                    int hist, hipStream_t s);
 hipError_t launch_select1(const Batch& b, const Geom& g, hipStream_t s);
 bool walk_in_detect_supported(const Geom& g);
+// Dilation windows too wide for a staged tile: detect writes the row maxima
+// to each frame's tmp plane (Frame::tmp, H*W floats at least) first.
+bool detect_needs_plane(const Geom& g);
 hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_select2(const Batch& b, const Geom& g, hipStream_t s);

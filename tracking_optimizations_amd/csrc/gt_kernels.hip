@@ -1094,6 +1094,126 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
     }
 }
 
+// Dilation windows too wide for the staged tile of k_detect_generic (its
+// (64 + kd - 1)^2 + (64 + kd - 1) * 64 floats exceed 60 KB from kd = 61, i.e.
+// separation 44): the row maxima of max_rows.wgsl are written to a plane
+// (the frame's tmp buffer) by one kernel, and the column test of
+// walk_cols.wgsl:23-34 reads that plane in a second.  Each output takes kd
+// loads (coalesced across the wave, reused through the caches), as the
+// shaders do.
+__global__ __launch_bounds__(256) void k_dilate_rows(Batch b, int KD, int a) {
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t f = b.tile_frame[item];
+    const Frame fr = b.frames[f];
+    const uint32_t local = item - fr.tile_base;
+    const int H = (int)fr.H, W = (int)fr.W;
+    const int x0 = (int)(local % fr.tx) * TILE_W;
+    const int y0 = (int)(local / fr.tx) * TILE_H;
+    const int c = threadIdx.x & 63, k = threadIdx.x >> 6;
+    const int v = x0 + c;
+    if (v >= W) return;
+    const float* P = fr.P;
+    float* RM = fr.tmp;
+    const int j0 = max(0, a - v), j1 = min(KD, W + a - v);   // in-bounds taps v - a + j
+    for (int m = 0; m < 16; ++m) {
+        const int u = y0 + k * 16 + m;
+        if (u >= H) break;
+        const float* row = P + (size_t)u * W;
+        float mx = __uint_as_float(NEG_INF_BITS);
+        for (int j = j0; j < j1; ++j) mx = fmaxf(mx, row[v - a + j]);
+        RM[(size_t)u * W + v] = mx;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_detect_plane(Batch b, DetParams dp) {
+    __shared__ uint64_t s_mask[TILE_H];
+    __shared__ uint32_t s_off[TILE_H];
+    __shared__ uint32_t s_lcnt;
+    if (threadIdx.x == 0) s_lcnt = 0;
+    __syncthreads();
+    const int KD = dp.kd, a = dp.a;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t f = b.tile_frame[item];
+    const Frame fr = b.frames[f];
+    const uint32_t local = item - fr.tile_base;
+    const int H = (int)fr.H, W = (int)fr.W;
+    const int x0 = (int)(local % fr.tx) * TILE_W;
+    const int y0 = (int)(local / fr.tx) * TILE_H;
+    const int tid = threadIdx.x, c = tid & 63, k = tid >> 6;
+    const int v = x0 + c;
+    const float* P = fr.P;
+    const float* RM = fr.tmp;
+    int32_t bin_lo = -1, bin_hi = -1;
+    if (dp.has_pct) {
+        bin_lo = b.selstate[f * 4 + 0];
+        bin_hi = b.selstate[f * 4 + 2];
+    }
+    uint32_t* list = b.list + (size_t)item * TILE_PX;
+    uint32_t cbits = 0;
+    for (int m = 0; m < 16; ++m) {
+        const int r = k * 16 + m;
+        const int u = y0 + r;
+        const bool inb = (u < H) && (v < W);
+        float center = 0.f;
+        bool cand = false;
+        if (inb) {
+            center = P[(size_t)u * W + v];
+            const int i0 = max(0, a - u), i1 = min(KD, H + a - u);   // in-bounds rows u - a + i
+            float cmx = __uint_as_float(NEG_INF_BITS);
+            for (int i = i0; i < i1; ++i) cmx = fmaxf(cmx, RM[(size_t)(u - a + i) * W + v]);
+            const bool interior = (u - dp.band >= 0) && (u + dp.band < H) &&
+                                  (v - dp.band >= 0) && (v + dp.band < W);
+            cand = interior && (center >= cmx) && !(dp.diw && center == 0.f);
+        }
+        const uint64_t msk = __ballot(cand);
+        if (c == 0) s_mask[r] = msk;
+        cbits |= (cand ? 1u : 0u) << m;
+        if (dp.has_pct) {
+            const uint32_t bits = __float_as_uint(center);
+            const int key = (int)pct_key(bits);
+            const bool match = inb && key != 0 && (key == bin_lo || key == bin_hi);
+            const uint64_t mm = __ballot(match);
+            if (mm) {
+                uint32_t basei = 0;
+                if (c == 0) basei = atomicAdd(&s_lcnt, (uint32_t)__popcll(mm));
+                basei = __shfl(basei, 0, 64);
+                if (match) list[basei + __popcll(mm & lanemask_lt(c))] = bits;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t cnt = (uint32_t)__popcll(s_mask[tid]);
+        const uint32_t inc = wave_incl_scan(cnt, tid);
+        s_off[tid] = inc - cnt;
+        if (tid == 63) {
+            b.tile_count[item] = inc;
+            if (dp.has_pct) b.tile_lcount[item] = s_lcnt;
+        }
+    }
+    __syncthreads();
+    uint16_t* slots = b.slots + (size_t)item * TILE_PX;
+    for (int m = 0; m < 16; ++m) {
+        if (cbits & (1u << m)) {
+            const int r = k * 16 + m;
+            const uint32_t rank = s_off[r] + (uint32_t)__popcll(s_mask[r] & lanemask_lt(c));
+            slots[rank] = (uint16_t)(r * TILE_W + c);
+        }
+    }
+}
+
+static bool detect_specialised(int kd);
+
+static size_t detect_generic_lds(int kd) {
+    const int kd1 = max(kd, 1);
+    return sizeof(float) * ((size_t)(TILE_H + kd1 - 1) * (TILE_W + kd1 - 1) +
+                            (size_t)(TILE_H + kd1 - 1) * TILE_W);
+}
+
+bool detect_needs_plane(const Geom& g) {
+    return !detect_specialised(g.kd) && detect_generic_lds(g.kd) > 60 * 1024;
+}
+
 template <int KD>
 static hipError_t launch_det_t(const Batch& b, const DetParams& dp, hipStream_t s) {
     switch (dp.walk_ri) {
@@ -1153,10 +1273,14 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
         case 21: return launch_det_t<21>(b, dp, s);
         default: break;
     }
-    const int kd1 = max(g.kd, 1);
-    const size_t shm = sizeof(float) * ((size_t)(TILE_H + kd1 - 1) * (TILE_W + kd1 - 1) +
-                                        (size_t)(TILE_H + kd1 - 1) * TILE_W);
-    if (shm > 60 * 1024) return hipErrorInvalidValue;
+    if (detect_needs_plane(g)) {
+        hipLaunchKernelGGL(k_dilate_rows, dim3(b.tiles), dim3(256), 0, s, b, g.kd, g.a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_detect_plane, dim3(b.tiles), dim3(256), 0, s, b, dp);
+        return hipGetLastError();
+    }
+    const size_t shm = detect_generic_lds(g.kd);
     hipLaunchKernelGGL(k_detect_generic, dim3(b.tiles), dim3(256), shm, s, b, dp);
     return hipGetLastError();
 }
