@@ -216,6 +216,32 @@ gt_status gt_locate_batch(gt_context* ctx, const gt_frame_desc* frames,
                           uint64_t n_frames, int dtype, const gt_params* p,
                           uint32_t flags, gt_result** out);
 
+/* No counterpart in the reference.  gt_locate_u16 / gt_locate_u8 /
+ * gt_locate_f32 with GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE, for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the call allocates no device memory and
+ * calls no allocating library routine.  Every workspace (frame tables,
+ * bandpassed frames, candidate slots and records, counters) lives in one
+ * fixed device array of GT_NOALLOC_SCRATCH_BYTES per device, held from
+ * library load until the process ends; frames are located in batches that
+ * fit it, and a single frame that needs more fails with GT_E_INVALID (about
+ * 113 MB for a 2048 x 2048 frame at d = 11 with percentile; the
+ * characterisation columns need more per frame).
+ *   frames   device pointer, n_frames x h x w of dtype (0 u16, 1 f32, 2 u8);
+ *   rows     device pointer, rows_capacity x ncols float32 (column_names
+ *            order), written row-major;
+ *   cand     device pointer, rows_capacity int64 candidate pixel indices;
+ *   n_rows   rows found.  When it exceeds rows_capacity the call returns
+ *            GT_E_INVALID and only the first rows_capacity rows are written.
+ * Rows equal those of gt_locate_* for the same input.  search_range is not
+ * supported here (the linker allocates).  Calls on one device share its
+ * scratch: calls that overlap in time on the same device (two threads, two
+ * contexts) race; calls in sequence do not. */
+#define GT_NOALLOC_SCRATCH_BYTES (128u << 20)
+gt_status gt_locate_noalloc(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
+                            uint32_t h, uint32_t w, const gt_params* p, float* rows,
+                            int64_t* cand, uint64_t rows_capacity, uint64_t* n_rows);
+
 /* --- results ------------------------------------------------------------ */
 
 uint64_t gt_result_rows(const gt_result* r);

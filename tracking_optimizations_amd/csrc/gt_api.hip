@@ -25,6 +25,14 @@
 
 using namespace gt;
 
+// Scratch of gt_locate_noalloc: every workspace of that entry point lives in
+// this array, which each device holds from module load until the process
+// ends.  Host code never passes `scratch` itself (its name is a host-side
+// stand-in, not a device address): gt_context_create takes the device
+// address with hipGetSymbolAddress, with the device current, and the entry
+// point uses that.
+static __device__ __align__(256) unsigned char scratch[GT_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 thread_local std::string t_err;
@@ -393,6 +401,14 @@ struct gt_context {
     // pinned host buffers of gt_locate_file (two chunks), kept for reuse
     void* pinned[2] = {nullptr, nullptr};
     size_t pinned_bytes = 0;
+    // gt_locate_noalloc: this device's address of `scratch` (null when the
+    // symbol lookup failed) and the host frame tables of one batch, sized at
+    // creation so that the entry point itself allocates nothing
+    void* scratch = nullptr;
+    static constexpr size_t kNaFrames = 1024, kNaBases = 16384;
+    std::vector<Frame> na_ftab;
+    std::vector<uint32_t> na_bases;
+    hipEvent_t ev_na = nullptr;
 };
 
 // Host result buffers.  Blocks of 1 MiB and more are kept when a result is
@@ -1343,12 +1359,24 @@ gt_status gt_context_create(int device, gt_context** out) {
         ok = ok && hipStreamCreateWithFlags(&c->streams[i], hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&c->ev_scan[i], hipEventDisableTiming) == hipSuccess;
     ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess &&
-         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) == hipSuccess;
+         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&c->ev_na, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         delete c;
         return fail(GT_E_HIP, "stream/event creation failed");
     }
     c->stream = c->streams[0];
+    // gt_locate_noalloc's scratch on this device (the device is current)
+    void* sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se == hipSuccess && sp) {
+        c->scratch = sp;
+    } else {
+        (void)hipGetLastError();
+        c->scratch = nullptr;   // gt_locate_noalloc reports the failure
+    }
+    c->na_ftab.resize(gt_context::kNaFrames);
+    c->na_bases.resize(gt_context::kNaBases);
     *out = c;
     return GT_OK;
 }
@@ -1370,6 +1398,7 @@ gt_status gt_context_destroy(gt_context* ctx) {
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
     (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_na) (void)hipEventDestroy(ctx->ev_na);
     for (int i = 0; i < kStreams; ++i) {
         (void)hipEventDestroy(ctx->ev_scan[i]);
         (void)hipStreamDestroy(ctx->streams[i]);
@@ -1707,6 +1736,213 @@ static gt_status locate_uniform(gt_context* ctx, const void* frames, uint64_t n,
     return run_locate(ctx, v, in_type, p, flags, out);
 }
 
+// gt_locate_noalloc: the launch sequence of run_locate's batched path on one
+// stream, with every workspace carved out of the device's `scratch` array
+// and the rows written into the caller's device buffers.  Nothing is
+// allocated: the host frame tables live in the context (sized at creation),
+// the parameter block and the counters in the scratch.
+static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type, uint64_t n,
+                                uint32_t h, uint32_t w, const gt_params* p, float* d_rows,
+                                int64_t* d_cand, uint64_t rows_cap, uint64_t* n_rows) {
+    if (!ctx) return fail(GT_E_INVALID, "context is NULL");
+    if (!n_rows) return fail(GT_E_INVALID, "n_rows is NULL");
+    *n_rows = 0;
+    if (n && !frames) return fail(GT_E_INVALID, "frames is NULL");
+    if (rows_cap && (!d_rows || !d_cand)) return fail(GT_E_INVALID, "rows or cand is NULL");
+    GT_TRY(check_params(p, h, w));
+    if (p->has_search_range)
+        return fail(GT_E_NOT_IMPLEMENTED, "gt_locate_noalloc does not link (search_range)");
+    if (!ctx->scratch)
+        return fail(GT_E_HIP, "the no-allocation scratch has no device address on device %d",
+                    ctx->device);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const bool integer_input = in_type != IN_F32;
+    Geom g;
+    GT_TRY(make_geom(p, integer_input, g));
+    g.walk_in_detect = ctx->fuse_walk && walk_in_detect_supported(g) ? 1 : 0;
+    const size_t es = elem_size(in_type);
+    const bool vec = w % 4 == 0 && ((uintptr_t)frames) % (4 * es) == 0;
+    bool generic;
+    {
+        Batch probe;
+        std::memset(&probe, 0, sizeof(probe));
+        probe.in_type = in_type;
+        probe.vec = vec ? 1 : 0;
+        generic = !bandpass_fast_path(probe, g);
+    }
+    const bool tmp_planes = generic || detect_needs_plane(g);
+    const int seg = ctx->seg_set ? ctx->seg_rows : 128;
+    const FrameSizes fs = sizes_of(h, w, seg);
+
+    // scratch layout for nb frames, every part 256-byte aligned
+    struct Layout {
+        size_t geom, counters, ftab, bases, P, tmp, list, slots, rec, keep, tile3, perframe, end;
+        uint64_t tiles, k1, nbases;
+    };
+    auto layout = [&](uint64_t nb) {
+        Layout L{};
+        size_t o = 0;
+        auto take = [&](size_t bytes) { const size_t at = o; o += al256(bytes); return at; };
+        L.tiles = nb * fs.tiles;
+        L.k1 = nb * fs.k1_items;
+        L.nbases = 2 * (nb + 1) + L.k1 + L.tiles;
+        const uint64_t slots = L.tiles * TILE_PX;
+        L.geom = take(sizeof(Geom));
+        L.counters = take(16);
+        L.ftab = take(nb * sizeof(Frame));
+        L.bases = take(L.nbases * 4);
+        L.P = take(nb * fs.px * 4);
+        L.tmp = tmp_planes ? take(nb * fs.px * 8) : 0;
+        L.list = g.has_pct ? take(slots * 4) : 0;
+        L.slots = take(slots * 2);
+        L.rec = take(slots * 4 * (uint64_t)rec_planes(g));
+        L.keep = take(slots);
+        L.tile3 = take(al256(L.tiles * 4) * 4 + 2 * al256(L.tiles * 4 * TILE_H));
+        L.perframe = take(al256(nb * HIST_BINS * 4) + 3 * al256(nb * 4) + al256(nb * 16) +
+                          2 * al256(nb * 8));
+        L.end = o;
+        return L;
+    };
+    auto fits = [&](uint64_t nb) {
+        const Layout L = layout(nb);
+        return L.end <= (size_t)GT_NOALLOC_SCRATCH_BYTES && nb <= gt_context::kNaFrames &&
+               L.nbases <= gt_context::kNaBases;
+    };
+    if (n == 0) return GT_OK;
+    if (!fits(1))
+        return fail(GT_E_INVALID,
+                    "a %ux%u frame needs %zu bytes of scratch; gt_locate_noalloc holds %u",
+                    h, w, layout(1).end, (unsigned)GT_NOALLOC_SCRATCH_BYTES);
+    uint64_t nbmax = 1;
+    while (nbmax < n && fits(nbmax * 2)) nbmax *= 2;
+    {
+        uint64_t lo = nbmax, hi = std::min<uint64_t>(n, nbmax * 2);   // fits(lo), maybe not hi
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi + 1) / 2;
+            if (fits(mid)) lo = mid; else hi = mid - 1;
+        }
+        nbmax = lo;
+    }
+    const Layout L = layout(nbmax);
+    char* const base = (char*)ctx->scratch;
+    Geom* d_geom = (Geom*)(base + L.geom);
+    uint64_t* counters = (uint64_t*)(base + L.counters);
+    hipStream_t st = ctx->streams[0];
+    HIP_TRY(hipMemcpyAsync(d_geom, &g, sizeof(Geom), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill_u64(counters, 2, 0, st));
+    Frame* ftab = ctx->na_ftab.data();
+    uint32_t* bases = ctx->na_bases.data();
+    for (uint64_t b0 = 0; b0 < n; b0 += nbmax) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(nbmax, n - b0);
+        if (b0 > 0) HIP_TRY(hipEventSynchronize(ctx->ev_na));   // host tables copied out
+        // frame tables: k1 bases [nb + 1], tile bases [nb + 1], frame of every
+        // bandpass item, frame of every detect tile (batch-relative)
+        uint32_t* k1b = bases;
+        uint32_t* tb = bases + nb + 1;
+        uint32_t* k1f = tb + nb + 1;
+        uint32_t* tf = k1f + (uint64_t)nb * fs.k1_items;
+        for (uint32_t fi = 0; fi < nb; ++fi) {
+            Frame& F = ftab[fi];
+            std::memset(&F, 0, sizeof(F));
+            F.in = (const char*)frames + (b0 + fi) * fs.px * es;
+            F.P = (float*)(base + L.P) + (uint64_t)fi * fs.px;
+            F.tmp = tmp_planes ? (float*)(base + L.tmp) + 2 * (uint64_t)fi * fs.px : nullptr;
+            F.H = h;
+            F.W = w;
+            F.k1_tx = fs.k1_tx;
+            F.k1_ty = fs.k1_ty;
+            F.tx = fs.tx;
+            F.ty = fs.ty;
+            F.tile_base = fi * fs.tiles;
+            F.k1_base = fi * fs.k1_items;
+            F.slot_base = (uint64_t)F.tile_base * TILE_PX;
+            F.frame_index = (int64_t)(b0 + fi);
+            k1b[fi] = F.k1_base;
+            tb[fi] = F.tile_base;
+            for (uint32_t i = 0; i < fs.k1_items; ++i) k1f[F.k1_base + i] = fi;
+            for (uint32_t i = 0; i < fs.tiles; ++i) tf[F.tile_base + i] = fi;
+        }
+        k1b[nb] = nb * fs.k1_items;
+        tb[nb] = nb * fs.tiles;
+        const uint64_t nbases = 2 * ((uint64_t)nb + 1) + (uint64_t)nb * (fs.k1_items + fs.tiles);
+        Frame* d_ftab = (Frame*)(base + L.ftab);
+        uint32_t* d_bases = (uint32_t*)(base + L.bases);
+        HIP_TRY(hipMemcpyAsync(d_ftab, ftab, nb * sizeof(Frame), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_bases, bases, nbases * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(ctx->ev_na, st));
+
+        Batch b;
+        std::memset(&b, 0, sizeof(b));
+        b.frames = d_ftab;
+        b.k1_bases = d_bases;
+        b.tile_bases = d_bases + nb + 1;
+        b.k1_frame = b.tile_bases + nb + 1;
+        b.tile_frame = b.k1_frame + (uint64_t)nb * fs.k1_items;
+        b.nframes = nb;
+        b.k1_items = nb * fs.k1_items;
+        b.tiles = nb * fs.tiles;
+        b.in_type = in_type;
+        b.vec = vec ? 1 : 0;
+        char* t3 = base + L.tile3;
+        const size_t t3s = al256(L.tiles * 4);
+        b.tile_count = (uint32_t*)t3;
+        b.tile_keep = (uint32_t*)(t3 + t3s);
+        b.tile_row_off = (uint32_t*)(t3 + 2 * t3s);
+        b.tile_lcount = (uint32_t*)(t3 + 3 * t3s);
+        b.rowk = (uint32_t*)(t3 + 4 * t3s);
+        b.rowoff = (uint32_t*)(t3 + 4 * t3s + al256(L.tiles * 4 * TILE_H));
+        char* pf = base + L.perframe;
+        b.hist1 = g.has_pct ? (uint32_t*)pf : nullptr;
+        pf += al256(nbmax * HIST_BINS * 4);
+        b.list = g.has_pct ? (uint32_t*)(base + L.list) : nullptr;
+        b.frame_rows = (uint32_t*)pf; pf += al256(nbmax * 4);
+        b.frame_cands = (uint32_t*)pf; pf += al256(nbmax * 4);
+        pf += al256(nbmax * 4);
+        b.selstate = (int32_t*)pf; pf += al256(nbmax * 16);
+        b.thr = (double*)pf; pf += al256(nbmax * 8);
+        b.frame_row_base = (uint64_t*)pf;
+        b.slots = (uint16_t*)(base + L.slots);
+        b.rec = (float*)(base + L.rec);
+        b.keep = (uint8_t*)(base + L.keep);
+        b.slot_cap = L.tiles * TILE_PX;
+        b.cand_total = counters;
+        b.row_carry = counters + 1;
+        b.rows = d_rows;
+        b.rows_cand = d_cand;
+        b.rows_cap = rows_cap;
+
+        HIP_TRY(launch_zero_batch(b, st));
+        if (generic)
+            HIP_TRY(launch_bandpass_generic(b, d_geom, h * w, g.has_pct, st));
+        else
+            HIP_TRY(launch_bandpass(b, g, d_geom, seg, st));
+        if (g.has_pct) HIP_TRY(launch_select1(b, g, st));
+        HIP_TRY(launch_detect(b, g, d_geom, st));
+        if (g.has_pct) HIP_TRY(launch_select2(b, g, st));
+        HIP_TRY(launch_refine(b, g, d_geom, st));
+        HIP_TRY(launch_dedup(b, g, d_geom, st));
+        HIP_TRY(launch_tile_scan(b, fs.ty, st));
+        HIP_TRY(launch_frame_scan(b, st));
+        HIP_TRY(launch_emit(b, g, d_geom, st));
+        if (g.raw_cols) HIP_TRY(launch_raw(b, g, d_geom, st));
+    }
+    uint64_t cnt[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(cnt, counters, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *n_rows = cnt[1];
+    ctx->last = gt_timing{};
+    ctx->last.frames = n;
+    ctx->last.pixels = n * fs.px;
+    ctx->last.candidates = cnt[0];
+    ctx->last.rows = cnt[1];
+    if (cnt[1] > rows_cap)
+        return fail(GT_E_INVALID,
+                    "%llu rows found but rows_capacity is %llu; only the first %llu were written",
+                    (unsigned long long)cnt[1], (unsigned long long)rows_cap,
+                    (unsigned long long)rows_cap);
+    return GT_OK;
+}
+
 gt_status gt_locate_u16(gt_context* ctx, const uint16_t* frames, uint64_t n_frames, uint32_t h,
                         uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
     return guarded([&]() -> gt_status {
@@ -1725,6 +1961,20 @@ gt_status gt_locate_f32(gt_context* ctx, const float* frames, uint64_t n_frames,
                         uint32_t w, const gt_params* p, uint32_t flags, gt_result** out) {
     return guarded([&]() -> gt_status {
     return locate_uniform(ctx, frames, n_frames, h, w, IN_F32, p, flags, out);
+    });
+}
+
+gt_status gt_locate_noalloc(gt_context* ctx, const void* frames, int dtype, uint64_t n_frames,
+                            uint32_t h, uint32_t w, const gt_params* p, float* rows,
+                            int64_t* cand, uint64_t rows_capacity, uint64_t* n_rows) {
+    return guarded([&]() -> gt_status {
+    int in_type;
+    if (dtype == 0) in_type = IN_U16;
+    else if (dtype == 1) in_type = IN_F32;
+    else if (dtype == 2) in_type = IN_U8;
+    else return fail(GT_E_INVALID, "dtype %d (0 u16, 1 f32, 2 u8)", dtype);
+    return locate_noalloc(ctx, frames, in_type, n_frames, h, w, p, rows, cand, rows_capacity,
+                          n_rows);
     });
 }
 
