@@ -13,7 +13,10 @@ entry is for the dominant kernel, measured with HIP events on the stream the
 library launches it on, in one extra step that runs the batches on one stream
 (so that a kernel's duration is not stretched by the next batch's kernels);
 the CPU baseline is the oracle (the CPU restatement of the reference
-pipeline) on a bounded sample of the same frames.
+pipeline, compiled with -march=native on the host it runs on) on the first 64
+frames, on one thread and on the 16 threads of one GPU's CPU share.  The
+same line carries the single-frame latency of the 512x512 workload (plain GPU
+path and CPU oracle) and the uneven 4096-frame batch, batched and plain.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -45,11 +48,43 @@ def algorithmic_bytes(h, w, n_cand, n_rows, ncols, rint):
     return 10.0 * h * w + n_cand * (4.0 + 4.0 * (2 * rint + 1) ** 2) + 4.0 * ncols * n_rows
 
 
-def cpu_baseline(frames_host: np.ndarray, kw: dict, threads: int):
-    """Oracle frames/s over the sample, frames spread over `threads` threads
-    (ctypes releases the GIL, so the threads run in parallel)."""
-    from concurrent.futures import ThreadPoolExecutor
+_NATIVE_ORACLE = None
+
+
+def native_oracle():
+    """The oracle compiled for this host's CPU (gcc -O3 -march=native
+    -ffp-contract=off -fopenmp) into a temporary directory: the CPU baseline
+    is timed with the build its host would use."""
+    global _NATIVE_ORACLE
     from oracle import oracle
+    if _NATIVE_ORACLE is None:
+        import tempfile
+        _NATIVE_ORACLE = oracle.use_native_build(tempfile.mkdtemp(prefix="gt_oracle_native_"))
+    return oracle
+
+
+def host_cpu() -> dict:
+    """CPUs this process may run on, the machine's total and its model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpus_available": len(os.sched_getaffinity(0)), "cpus_machine": os.cpu_count(),
+            "model": model}
+
+
+def cpu_baseline(frames_host: np.ndarray, kw: dict, threads: int):
+    """Oracle frames/s over the sample, frames spread over `threads` threads,
+    one frame per thread at a time (ctypes releases the GIL, so the threads
+    run in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+    oracle = native_oracle()
+    oracle.set_threads(1)
     p = oracle.Params(**kw)
     n = len(frames_host)
     f32 = [frames_host[i].astype(np.float32) for i in range(n)]
@@ -58,10 +93,116 @@ def cpu_baseline(frames_host: np.ndarray, kw: dict, threads: int):
         return len(oracle.locate_frame(f32[i], p, frame_index=i)[0])
 
     t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as ex:
-        rows = sum(ex.map(one, range(n)))
+    if threads == 1:
+        rows = sum(one(i) for i in range(n))
+    else:
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            rows = sum(ex.map(one, range(n)))
     dt = time.perf_counter() - t0
     return n / dt, rows, dt
+
+
+def c1_latency(ctx, threads: int, repeats: int = 30) -> dict:
+    """Single-frame locate of the 512x512 frame (200 spots, sigma 1.5, peaks
+    U(500, 2000), background 100 + Poisson; diameter 9, separation 9,
+    smoothing 9, percentile 64, minmass 1000): median wall time of a warm
+    call on the plain per-frame GPU path (frame in HBM, rows left in HBM;
+    and numpy frame in, rows back on the host) and of the CPU oracle on one
+    thread and on `threads` threads (the frame's rows split over them)."""
+    import ctypes
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import locate, make_params
+    L = _lib.load()
+    spec = synth.single_512(seed=1)
+    frame = synth.generate(ctx, spec)
+    host = frame.to_host(np.uint16, (1, 512, 512))
+    kw = dict(diameter=9, separation=9, minmass=1000.0, percentile=64.0)
+    p = make_params(**kw)
+
+    def dev_call():
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(frame.ptr), 1, 512, 512,
+                                   ctypes.byref(p), _lib.GT_FRAMES_ON_DEVICE |
+                                   _lib.GT_RESULT_ON_DEVICE | _lib.GT_PLAIN, ctypes.byref(out)))
+        _lib.Result(out)
+
+    def host_call():
+        locate(host, p, plain=True, ctx=ctx)
+
+    def median_ms(fn, n):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return 1000.0 * float(np.median(ts))
+
+    out = {"frame": "512x512 uint16, 200 spots", "params": kw, "repeats": repeats,
+           "gpu_plain_device_ms": median_ms(dev_call, repeats),
+           "gpu_plain_host_ms": median_ms(host_call, repeats)}
+    oracle = native_oracle()
+    op = oracle.Params(**kw)
+    f32 = host[0].astype(np.float32)
+    rows = len(oracle.locate_frame(f32, op)[0])
+    old = oracle.set_threads(1)
+    out["cpu_1_thread_ms"] = median_ms(lambda: oracle.locate_frame(f32, op), max(20, repeats // 2))
+    oracle.set_threads(threads)
+    out["cpu_threads"] = threads
+    out["cpu_all_threads_ms"] = median_ms(lambda: oracle.locate_frame(f32, op),
+                                          max(20, repeats // 2))
+    oracle.set_threads(old)
+    out["rows"] = rows
+    frame.free()
+    return out
+
+
+def uneven_batch(ctx, steps: int, n_frames: int = 4096) -> dict:
+    """The uneven batch (4096 frames: 40 % 256x256, 40 % 512x1024, 20 %
+    2048x2048, shuffled; one spot per 800 px^2, sigma U(1.2, 2.5), background
+    U(50, 300)) device-resident, located in one gt_locate_batch call per step
+    on the batched path, then once on the plain per-frame path."""
+    import ctypes
+    from tracking_optimizations_amd import _lib, synth
+    from tracking_optimizations_amd.gpu_tracking import make_params
+    kw = dict(diameter=11, separation=11, smoothing_size=11, percentile=64, minmass=0.0)
+    params = make_params(**kw)
+    L = _lib.load()
+    movie = synth.UnevenMovie(ctx, n_frames, seed=3)
+    descs = movie.frame_descs()
+    ctx.synchronize()
+
+    def run(flags):
+        out = ctypes.c_void_p()
+        _lib.check(L.gt_locate_batch(ctx.handle, descs, n_frames, 0, ctypes.byref(params), flags,
+                                     ctypes.byref(out)))
+        r = _lib.Result(out)
+        t = ctx.timing()
+        del r
+        return t
+
+    bflags = _lib.GT_FRAMES_ON_DEVICE | _lib.GT_RESULT_ON_DEVICE
+    run(bflags)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = run(bflags)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    run(bflags | _lib.GT_PLAIN)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    run(bflags | _lib.GT_PLAIN)
+    ctx.synchronize()
+    pel = time.perf_counter() - t0
+    fps = n_frames * steps / el
+    res = {"frames": n_frames, "steps": steps, "frames_per_s": fps,
+           "pixels_per_s": movie.pixels * steps / el, "plain_frames_per_s": n_frames / pel,
+           "batched_over_plain": fps * pel / n_frames, "rows": st["rows"],
+           "candidates": st["candidates"]}
+    movie.free()
+    return res
 
 
 def main() -> int:
@@ -72,7 +213,9 @@ def main() -> int:
     ap.add_argument("--frames", type=int, default=1000)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--batch", type=int, default=0, help="frames per launch (0 = automatic)")
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch figures of the default line")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
@@ -263,19 +406,36 @@ def main() -> int:
         log(f"plain per-frame path: {nf / pe:.0f} frames/s ({1000 * pe / nf:.3f} ms/frame); "
             f"batched / plain = {fps / (nf / pe):.2f}x")
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_batch(ctx, args.steps)
+        log(f"uneven batch (4096 frames): {uneven['frames_per_s']:.0f} frames/s batched, "
+            f"{uneven['plain_frames_per_s']:.0f} plain, {uneven['batched_over_plain']:.2f}x")
+
     cpu = None
+    c1 = None
     if not args.no_cpu and args.cpu_sample > 0:
+        hc = host_cpu()
+        # the CPU share of one GPU on the box: 16 threads, or fewer CPUs
+        threads = max(1, min(hc["cpus_available"], 16))
+        c1 = c1_latency(ctx, threads)
+        log(f"C1 single frame: GPU plain {c1['gpu_plain_device_ms']:.3f} ms (frame in HBM), "
+            f"{c1['gpu_plain_host_ms']:.3f} ms (numpy in, rows out); CPU oracle "
+            f"{c1['cpu_1_thread_ms']:.1f} ms on 1 thread, {c1['cpu_all_threads_ms']:.1f} ms on "
+            f"{threads}")
         ns = min(args.cpu_sample, nf)
         sample = movie.to_host(np.uint16, (ns, H, W))
-        cores = len(os.sched_getaffinity(0))
-        threads = max(1, min(cores, 16))
         okw = dict(diameter=11, separation=11, smoothing_size=11, minmass=0.0,
                    percentile=None if args.no_percentile else 64.0)
+        c1fps, _, c1dt = cpu_baseline(sample, okw, 1)
         cfps, crows, cdt = cpu_baseline(sample, okw, threads)
-        log(f"cpu oracle: {cfps:.2f} frames/s over {ns} frames on {threads} threads ({cdt:.1f} s)")
+        log(f"cpu oracle: {cfps:.2f} frames/s over {ns} frames on {threads} threads ({cdt:.1f} s); "
+            f"{c1fps:.2f} frames/s on 1 thread ({c1dt:.1f} s); host {hc}")
         cpu = {"value": cfps, "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"first {ns} frames of the same movie, oracle binary32 mode, "
-                         f"{threads} threads (one frame per thread)"}
+               "sample": f"first {ns} frames of the same movie, oracle binary32 mode built "
+                         f"gcc -O3 -march=native on this host, {threads} threads (one frame "
+                         f"per thread)",
+               "single_thread_frames_per_s": c1fps, "host": hc}
 
     line = {
         "metric": "frames/s at 2048^2 uint16 (~5k spots) batched vs plain; HBM GB/s % of peak",
@@ -306,6 +466,8 @@ def main() -> int:
         "tuning": tuning,
         "plain": plain,
         "batched_over_plain": (fps / plain["frames_per_s"]) if plain else None,
+        "uneven": uneven,
+        "c1_latency": c1,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -37,6 +37,16 @@ int32_t orc_ncols(const orc_params* p) {
     return 4 + (p->cpu_processed ? 3 : 0) + (p->characterize ? 4 : 0);
 }
 
+/* Threads over the rows of one frame in the bandpass and local-maximum
+ * passes (orc_set_threads); 1: serial. */
+static int orc_threads = 1;
+
+int orc_set_threads(int n) {
+    int old = orc_threads;
+    orc_threads = n > 0 ? n : 1;
+    return old;
+}
+
 /* Planted mistake of the characterisation (orc_set_mutation): ORC_MUT_NONE
  * everywhere but in the tests that show their checks catch each one. */
 static int orc_mut = ORC_MUT_NONE;

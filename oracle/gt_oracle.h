@@ -115,6 +115,10 @@ int orc_characterise_at(const float* P, const float* raw, int32_t H, int32_t W,
                         const orc_params* p, int32_t u, int32_t v, float walked_mass,
                         float* cpu3, float* gpu4);
 
+/* Threads over the rows of one frame (bandpass and local-maximum passes);
+ * default 1.  Results do not depend on it.  Returns the previous value. */
+int orc_set_threads(int n);
+
 /* Planted mistakes, for the tests that show the characterisation checks
  * catch them (never set outside those tests).  Returns the previous value. */
 enum {

@@ -55,6 +55,24 @@ def build() -> str:
     return _LIB_PATH
 
 
+def use_native_build(out_dir: str) -> str:
+    """Compile the oracle for this machine's CPU (ARCH=-march=native) into
+    out_dir and load that build instead of oracle/_build's: the CPU baseline
+    of bench.py runs on the host it is timed on.  Returns the library path."""
+    global _lib, _LIB_PATH
+    out_dir = os.path.abspath(out_dir)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE, "ARCH=-march=native", f"OUT={out_dir}"],
+                   check=True)
+    _LIB_PATH = os.path.join(out_dir, "liboracle.so")
+    _lib = None
+    return _LIB_PATH
+
+
+def set_threads(n: int) -> int:
+    """Threads over the rows of one frame (1: serial); returns the previous."""
+    return lib().orc_set_threads(int(n))
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -100,6 +118,8 @@ def lib():
         L.orc_characterise_at.argtypes = [fp, fp, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.POINTER(OrcParams), ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_float, fp, fp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
         L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_set_mutation.restype = ctypes.c_int
         _lib = L

@@ -319,3 +319,21 @@ def test_binary32_error_against_binary64():
     assert rel(8) < 1e-5, rel(8)   # raw_mass_gpu
     for j in (5, 10):         # ecc, ecc_gpu
         assert ab(j) < 1e-3, (j, ab(j))
+
+
+def test_threads_do_not_change_rows():
+    """orc_set_threads splits a frame's rows over threads (the all-cores CPU
+    baseline); rows and their order are those of the serial run."""
+    rng = np.random.default_rng(12)
+    img = rng.poisson(120, (300, 260)).astype(np.float32)
+    for (cy, cx) in rng.uniform(15, 245, (25, 2)):
+        img += _spot((300, 260), cy, cx, 1200.0, 1.8, bg=0.0)
+    p = O.Params(diameter=9, separation=9, percentile=64, cpu_processed=True, characterize=True)
+    ref = O.locate_frame(img, p)
+    for n in (2, 3, 7):
+        old = O.set_threads(n)
+        try:
+            got = O.locate_frame(img, p)
+        finally:
+            O.set_threads(old)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), n
