@@ -131,6 +131,15 @@ gt_status gt_column_names(const gt_params* p, const char** names,
                           const char** dtypes, uint32_t cap, uint32_t* ncols);
 
 const char* gt_last_error(void);
+
+/* Test hook; no counterpart in the reference, which panics when an
+ * allocation fails (execute_gpu.rs:497-499, gpu_setup.rs:201,208): from now
+ * on, the after-th device allocation (kind 0: the library's workspaces and
+ * result buffers) or host result allocation (kind 1) made by the library
+ * fails, once, as if out of memory, so that callers can check that the call
+ * returns GT_E_OOM and the context keeps working.  after = 0 disarms.
+ * Process-wide. */
+gt_status gt_debug_inject_alloc_failure(int kind, uint32_t after);
 const char* gt_version(void);
 
 /* Number of visible HIP devices (0 when there is none; never fails). */

@@ -68,6 +68,14 @@ def use_native_build(out_dir: str) -> str:
     return _LIB_PATH
 
 
+def use_library(path: str) -> None:
+    """Load the oracle from another build of the same sources (for instance
+    the AddressSanitizer build of `make asan`)."""
+    global _lib, _LIB_PATH
+    _LIB_PATH = os.path.abspath(path)
+    _lib = None
+
+
 def set_threads(n: int) -> int:
     """Threads over the rows of one frame (1: serial); returns the previous."""
     return lib().orc_set_threads(int(n))

@@ -67,5 +67,66 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+ASAN_DIR = os.path.join(HERE, "_build_asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libgpu_tracking_asan.so")
+
+
+def build_asan(verbose: bool = False) -> str:
+    """The library with its host code instrumented by AddressSanitizer (device
+    code as usual, -fsanitize only after -Xarch_host), for CPU tests of the
+    host paths (parameters, column names, movie-file decoding, argument and
+    error handling) under clang's ASan runtime.  gt_kernels.hip, whose host
+    code is kernel launchers only, is linked from the regular build's object
+    (built first when missing)."""
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    flags = [f for f in FLAGS if f != "-O3"] + [
+        "-O1", "-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+    jobs = []
+    objs = []
+    for src in SOURCES:
+        if src == "gt_kernels.hip":
+            obj = os.path.join(objdir, "gt_kernels.o")
+            srcp = os.path.join(CSRC, src)
+            if not os.path.exists(obj) or os.path.getmtime(obj) < os.path.getmtime(srcp):
+                jobs.append(([HIPCC, *FLAGS, "-c", srcp, "-o", obj], src))
+        else:
+            obj = os.path.join(ASAN_DIR, src.replace(".hip", ".o"))
+            jobs.append(([HIPCC, *flags, "-c", os.path.join(CSRC, src), "-o", obj], src))
+        objs.append(obj)
+
+    def one(job):
+        cmd, src = job
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc (asan) failed on {src}")
+
+    with cf.ThreadPoolExecutor(max_workers=len(jobs) or 1) as ex:
+        list(ex.map(one, jobs))
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host",
+                        "-fsanitize=address", "-o", ASAN_LIB, *objs, "-lz"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link of the asan library failed")
+    if verbose:
+        print(f"built {ASAN_LIB}")
+    return ASAN_LIB
+
+
+def asan_runtime() -> str:
+    """clang's AddressSanitizer runtime matching hipcc (to preload)."""
+    import glob
+    cands = sorted(glob.glob(os.path.join(os.path.dirname(os.path.dirname(HIPCC)), "llvm", "lib",
+                                          "clang", "*", "lib", "linux",
+                                          "libclang_rt.asan-x86_64.so")))
+    cands += sorted(glob.glob("/opt/rocm/llvm/lib/clang/*/lib/linux/libclang_rt.asan-x86_64.so"))
+    if not cands:
+        raise FileNotFoundError("libclang_rt.asan-x86_64.so not found")
+    return cands[-1]
+
+
 if __name__ == "__main__":
     build()

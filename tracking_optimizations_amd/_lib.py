@@ -44,7 +44,7 @@ EXPORTS = [
     "gt_adjoint_sigma", "gt_adjoint_origin", "gt_adjoint_jacobian", "gt_adjoint_vjp",
     "gt_adjoint_free", "gt_context_set_profiling", "gt_context_stage_times", "gt_link",
     "gt_context_link_stats", "gt_file_open", "gt_file_info", "gt_file_channels", "gt_file_read",
-    "gt_file_close", "gt_locate_file", "gt_locate_noalloc",
+    "gt_file_close", "gt_locate_file", "gt_locate_noalloc", "gt_debug_inject_alloc_failure",
 ]
 
 STAGES = ["zero", "bandpass", "select1", "detect", "select2", "refine", "dedup", "scan", "emit"]
@@ -181,6 +181,8 @@ def load() -> ctypes.CDLL:
             f.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(GtParams),
                           ctypes.c_uint32, P(vp)]
             f.restype = st
+        L.gt_debug_inject_alloc_failure.argtypes = [ctypes.c_int, ctypes.c_uint32]
+        L.gt_debug_inject_alloc_failure.restype = st
         L.gt_locate_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
                                         ctypes.c_uint32, P(GtParams), vp, vp, ctypes.c_uint64,
                                         P(ctypes.c_uint64)]

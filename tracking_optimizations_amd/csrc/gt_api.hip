@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <new>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -77,6 +78,19 @@ gt_status guarded(F&& f) {
     }
 }
 
+// gt_debug_inject_alloc_failure: countdowns to a failing device (0) or host
+// result (1) allocation; 0 = disarmed.
+std::atomic<uint32_t> g_fail_after[2];
+
+// True when this allocation of `kind` is the one that must fail.
+bool injected_failure(int kind) {
+    uint32_t c = g_fail_after[kind].load();
+    while (c != 0) {
+        if (g_fail_after[kind].compare_exchange_weak(c, c - 1)) return c == 1;
+    }
+    return false;
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -87,6 +101,7 @@ struct DevBuf {
         p = nullptr;
         bytes = 0;
         n = (n + 255) & ~(size_t)255;
+        if (injected_failure(0)) return hipErrorOutOfMemory;
         hipError_t e = hipMalloc(&p, n);
         if (e == hipSuccess) bytes = n;
         else p = nullptr;
@@ -433,6 +448,7 @@ class HostBlockCache {
                 return p;
             }
         }
+        if (injected_failure(1)) throw std::bad_alloc();
         void* p = std::malloc(bytes ? bytes : 1);
         if (!p) throw std::bad_alloc();
         return p;
@@ -1249,6 +1265,12 @@ extern "C" {
 
 const char* gt_last_error(void) { return t_err.c_str(); }
 
+gt_status gt_debug_inject_alloc_failure(int kind, uint32_t after) {
+    if (kind != 0 && kind != 1) return fail(GT_E_INVALID, "kind %d (0 device, 1 host)", kind);
+    g_fail_after[kind].store(after);
+    return GT_OK;
+}
+
 const char* gt_version(void) { return "gpu_tracking-mi355x 0.1.0 (gfx950, abi 1)"; }
 
 int gt_device_count(void) {
@@ -1587,7 +1609,10 @@ static gt_status locate_staged(gt_context* ctx, const std::vector<FrameIn>& fram
             b = nullptr;
         }
         ctx->pinned_bytes = 0;
-        for (auto& b : ctx->pinned) HIP_TRY(hipHostMalloc(&b, max_chunk, hipHostMallocDefault));
+        for (auto& b : ctx->pinned) {
+            if (injected_failure(0)) return fail(GT_E_OOM, "hipHostMalloc: out of memory (injected)");
+            HIP_TRY(hipHostMalloc(&b, max_chunk, hipHostMallocDefault));
+        }
         ctx->pinned_bytes = max_chunk;
     }
     char* const buf[2] = {(char*)ctx->pinned[0], (char*)ctx->pinned[1]};

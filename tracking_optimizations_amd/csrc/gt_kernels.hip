@@ -694,6 +694,9 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                 // top-left corner of the disk's box in the staged area
                 const int br = u - y0 + a - RI, bc = v - x0 + a - RI;
                 float s0 = 0.f, s1 = 0.f, ms = 0.f;
+                // a tap with a zero coefficient leaves its sum unchanged:
+                // fma(d, 0, s) = s exactly (d is finite and s never -0,
+                // since the sums start at +0), so those FMAs are skipped
                 if (br >= 0 && br + BW <= LR && bc >= 0 && bc + BW <= LC) {
                     const float* box = L + br * LS + bc;
 #pragma unroll
@@ -702,8 +705,8 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                         for (int dj = -RI; dj <= RI; ++dj) {
                             if (di * di + dj * dj > RI * RI) continue;
                             const float d = box[(di + RI) * LS + (dj + RI)];
-                            s0 = __builtin_fmaf(d, (float)di, s0);
-                            s1 = __builtin_fmaf(d, (float)dj, s1);
+                            if (di != 0) s0 = __builtin_fmaf(d, (float)di, s0);
+                            if (dj != 0) s1 = __builtin_fmaf(d, (float)dj, s1);
                             ms = ms + d;
                         }
                 } else {
@@ -714,8 +717,8 @@ __device__ __forceinline__ void walk_tile(const Batch& b, const DetParams& dp,
                         for (int dj = -RI; dj <= RI; ++dj) {
                             if (di * di + dj * dj > RI * RI) continue;
                             const float d = row[dj];
-                            s0 = __builtin_fmaf(d, (float)di, s0);
-                            s1 = __builtin_fmaf(d, (float)dj, s1);
+                            if (di != 0) s0 = __builtin_fmaf(d, (float)di, s0);
+                            if (dj != 0) s1 = __builtin_fmaf(d, (float)dj, s1);
                             ms = ms + d;
                         }
                     }
