@@ -929,6 +929,7 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
     std::vector<Frame> ftab(frames.size());
     std::vector<uint32_t> bases_all;  // per batch: k1 bases [nb+1] then tile bases [nb+1]
     std::vector<size_t> bases_off(plan.size());
+    std::vector<uint32_t> dblocks_of(plan.size());
     for (size_t bi = 0; bi < plan.size(); ++bi) {
         const auto& bp = plan[bi];
         Workspace& wsb = ctx->ws[bi % nstreams];
@@ -973,6 +974,14 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             for (uint32_t i = k1v[fi]; i < k1v[fi + 1]; ++i) bases_all.push_back(fi);
         for (uint32_t fi = 0; fi + 1 < (uint32_t)tbv.size(); ++fi)
             for (uint32_t i = tbv[fi]; i < tbv[fi + 1]; ++i) bases_all.push_back(fi);
+        // top-left tile of every 2x2 block of tiles (dedup)
+        const size_t d0 = bases_all.size();
+        for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
+            const Frame& F = ftab[fi];
+            for (uint32_t ty = 0; ty < F.ty; ty += 2)
+                for (uint32_t tx = 0; tx < F.tx; tx += 2) bases_all.push_back(F.tile_base + ty * F.tx + tx);
+        }
+        dblocks_of[bi] = (uint32_t)(bases_all.size() - d0);
     }
     HIP_TRY(ctx->frames_tab.ensure(ftab.size() * sizeof(Frame)));
     HIP_TRY(ctx->bases.ensure(bases_all.size() * 4));
@@ -1029,6 +1038,8 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             b.tiles = bases_all[bases_off[bi] + nb + 1 + nb];
             b.k1_frame = b.tile_bases + nb + 1;
             b.tile_frame = b.k1_frame + b.k1_items;
+            b.dblock_tile = b.tile_frame + b.tiles;
+            b.dblocks = dblocks_of[bi];
             b.in_type = in_type;
             b.vec = vec ? 1 : 0;
             char* t3 = w.tile3.as<char>();
@@ -1798,6 +1809,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
     const bool tmp_planes = generic || detect_needs_plane(g);
     const int seg = ctx->seg_set ? ctx->seg_rows : 128;
     const FrameSizes fs = sizes_of(h, w, seg);
+    const uint64_t dblocks_pf = (uint64_t)((fs.ty + 1) / 2) * ((fs.tx + 1) / 2);  // dedup blocks
 
     // scratch layout for nb frames, every part 256-byte aligned
     struct Layout {
@@ -1810,7 +1822,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         auto take = [&](size_t bytes) { const size_t at = o; o += al256(bytes); return at; };
         L.tiles = nb * fs.tiles;
         L.k1 = nb * fs.k1_items;
-        L.nbases = 2 * (nb + 1) + L.k1 + L.tiles;
+        L.nbases = 2 * (nb + 1) + L.k1 + L.tiles + nb * dblocks_pf;
         const uint64_t slots = L.tiles * TILE_PX;
         L.geom = take(sizeof(Geom));
         L.counters = take(16);
@@ -1866,6 +1878,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         uint32_t* tb = bases + nb + 1;
         uint32_t* k1f = tb + nb + 1;
         uint32_t* tf = k1f + (uint64_t)nb * fs.k1_items;
+        uint32_t* dbt = tf + (uint64_t)nb * fs.tiles;
         for (uint32_t fi = 0; fi < nb; ++fi) {
             Frame& F = ftab[fi];
             std::memset(&F, 0, sizeof(F));
@@ -1886,10 +1899,14 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
             tb[fi] = F.tile_base;
             for (uint32_t i = 0; i < fs.k1_items; ++i) k1f[F.k1_base + i] = fi;
             for (uint32_t i = 0; i < fs.tiles; ++i) tf[F.tile_base + i] = fi;
+            uint32_t* db = dbt + (uint64_t)fi * dblocks_pf;
+            for (uint32_t ty = 0; ty < fs.ty; ty += 2)
+                for (uint32_t tx = 0; tx < fs.tx; tx += 2) *db++ = F.tile_base + ty * fs.tx + tx;
         }
         k1b[nb] = nb * fs.k1_items;
         tb[nb] = nb * fs.tiles;
-        const uint64_t nbases = 2 * ((uint64_t)nb + 1) + (uint64_t)nb * (fs.k1_items + fs.tiles);
+        const uint64_t nbases =
+            2 * ((uint64_t)nb + 1) + (uint64_t)nb * (fs.k1_items + fs.tiles + dblocks_pf);
         Frame* d_ftab = (Frame*)(base + L.ftab);
         uint32_t* d_bases = (uint32_t*)(base + L.bases);
         HIP_TRY(hipMemcpyAsync(d_ftab, ftab, nb * sizeof(Frame), hipMemcpyHostToDevice, st));
@@ -1903,6 +1920,8 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         b.tile_bases = d_bases + nb + 1;
         b.k1_frame = b.tile_bases + nb + 1;
         b.tile_frame = b.k1_frame + (uint64_t)nb * fs.k1_items;
+        b.dblock_tile = b.tile_frame + (uint64_t)nb * fs.tiles;
+        b.dblocks = (uint32_t)(nb * dblocks_pf);
         b.nframes = nb;
         b.k1_items = nb * fs.k1_items;
         b.tiles = nb * fs.tiles;

@@ -95,9 +95,11 @@ struct Batch {
     const uint32_t* tile_bases; // [nframes + 1]
     const uint32_t* k1_frame;   // [k1_items] frame of each bandpass item
     const uint32_t* tile_frame; // [tiles] frame of each detect tile
+    const uint32_t* dblock_tile;  // [dblocks] top-left tile of each 2x2 block of tiles (dedup)
     uint32_t nframes;
     uint32_t k1_items;
     uint32_t tiles;
+    uint32_t dblocks;
     int in_type;           // InType
     int vec;               // W % 4 == 0 and aligned frame bases: vector loads
     // per-tile arrays [tiles]

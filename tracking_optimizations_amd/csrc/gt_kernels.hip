@@ -1840,25 +1840,29 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 }
 
 // ---------------------------------------------------------------------------
-// K4 dedup: one wave per tile.  The valid records (mass != 0) of the tiles
-// that can hold a neighbour, and that lie within `separation` of the
-// bounding box of the tile's own valid positions, are staged in LDS as
-// 16-byte entries.  Each lane then tests its own candidate against every
-// staged record (execute_gpu.rs:258-284: drop when a record strictly within
-// `separation` has a larger mass, or an equal mass and a lower raster
-// index).  Neighbourhoods larger than the staging area are staged and tested
-// in rounds.
+// K4 dedup: one wave per block of 2x2 tiles (the block's tiles share one
+// chain of dependent loads and one staged neighbourhood).  The valid records
+// (mass != 0) of the tiles that can hold a neighbour, and that lie within
+// `separation` of the bounding box of the block's own valid positions, are
+// staged in LDS as 16-byte entries.  Each lane then tests its own candidate
+// against every staged record (execute_gpu.rs:258-284: drop when a record
+// strictly within `separation` has a larger mass, or an equal mass and a
+// lower raster index).  Neighbourhoods larger than the staging area are
+// staged and tested in rounds.  Outputs stay per tile: keep flags, kept
+// count and kept rows per pixel row.
 // ---------------------------------------------------------------------------
 constexpr int DD_CAP = 256;   // staged records per wave and round
 
-__global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__ g) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+void k_dedup(Batch b, const Geom* __restrict__ g) {
     // staged records, one 16-byte entry each: mass, y, x, raster index (bits)
     __shared__ float4 s_rec[4][DD_CAP];
-    __shared__ uint32_t s_kr[4][TILE_H];
+    __shared__ uint32_t s_kr[4][4][TILE_H];   // kept rows per block tile and pixel row
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
-    if (item >= b.tiles) return;
-    s_kr[wave][lane] = 0;
+    if (item >= b.dblocks) return;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s_kr[wave][q][lane] = 0;
     const uint64_t cap = b.slot_cap;
     const GLOBAL float* rmass = (const GLOBAL float*)(b.rec + REC_MASS * cap);
     const GLOBAL float* ry = (const GLOBAL float*)(b.rec + REC_Y * cap);
@@ -1868,13 +1872,28 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     const float r2 = rad * rad;
     const int rty = g->dedup_rty, rtx = g->dedup_rtx;
 
-    const uint32_t f = b.tile_frame[item];
+    const uint32_t top = b.dblock_tile[item];   // the block's top-left tile
+    const uint32_t f = b.tile_frame[top];
     const Frame fr = b.frames[f];
-    const uint32_t local = item - fr.tile_base;
+    const uint32_t local = top - fr.tile_base;
     const int W = (int)fr.W;
     const int tx = (int)(local % fr.tx), ty = (int)(local / fr.tx);
-    const uint32_t cnt = b.tile_count[item];
-    const size_t slot0 = (size_t)item * TILE_PX;
+    const bool hx = tx + 1 < (int)fr.tx, hy = ty + 1 < (int)fr.ty;
+    // tile q = 2 dy + dx of the block (absent ones hold no candidate)
+    const bool has[4] = {true, hx, hy, hx && hy};
+    uint32_t tq[4], cq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        tq[q] = top + (uint32_t)((q >> 1) * (int)fr.tx + (q & 1));
+        cq[q] = has[q] ? b.tile_count[tq[q]] : 0u;
+    }
+    const uint32_t e1 = cq[0], e2 = e1 + cq[1], e3 = e2 + cq[2], own_n = e3 + cq[3];
+    // own record j of the block: its tile q and slot
+    auto own_q = [&](uint32_t j) { return j < e1 ? 0 : (j < e2 ? 1 : (j < e3 ? 2 : 3)); };
+    auto own_slot = [&](uint32_t j, int q) {
+        const uint32_t first = q == 0 ? 0u : (q == 1 ? e1 : (q == 2 ? e2 : e3));
+        return (size_t)tq[q] * TILE_PX + (j - first);
+    };
     // when the walk ran in detect, every candidate walked and the percentile
     // test on its start value (refine's `P > thr`) is applied here
     const bool use_thr = g->walk_in_detect && g->has_pct;
@@ -1883,16 +1902,17 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
     auto ok = [&](float mass, size_t s) {
         return mass != 0.f && (!use_thr || (double)rcen[s] > thr);
     };
-    const int y_lo = max(0, ty - rty), y_hi = min((int)fr.ty - 1, ty + rty);
-    const int x_lo = max(0, tx - rtx), x_hi = min((int)fr.tx - 1, tx + rtx);
+    const int y_lo = max(0, ty - rty), y_hi = min((int)fr.ty - 1, ty + 1 + rty);
+    const int x_lo = max(0, tx - rtx), x_hi = min((int)fr.tx - 1, tx + 1 + rtx);
     const int nxr = x_hi - x_lo + 1;
     const int ntl = (y_hi - y_lo + 1) * nxr;
 
-    // bounding box of the tile's own valid positions
+    // bounding box of the block's own valid positions
     float bymin = INFINITY, bymax = -INFINITY, bxmin = INFINITY, bxmax = -INFINITY;
-    for (uint32_t i = lane; i < cnt; i += 64) {
-        if (ok(rmass[slot0 + i], slot0 + i)) {
-            const float y = ry[slot0 + i], x = rx[slot0 + i];
+    for (uint32_t j = lane; j < own_n; j += 64) {
+        const size_t sl = own_slot(j, own_q(j));
+        if (ok(rmass[sl], sl)) {
+            const float y = ry[sl], x = rx[sl];
             bymin = fminf(bymin, y); bymax = fmaxf(bymax, y);
             bxmin = fminf(bxmin, x); bxmax = fmaxf(bxmax, x);
         }
@@ -1903,11 +1923,14 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
         bxmin = fminf(bxmin, __shfl_xor(bxmin, o, 64));
         bxmax = fmaxf(bxmax, __shfl_xor(bxmax, o, 64));
     }
-    uint32_t kept = 0;
     if (!(bymin <= bymax)) {  // no valid own record
-        for (uint32_t i = lane; i < cnt; i += 64) b.keep[slot0 + i] = 0;
-        if (lane == 0) b.tile_keep[item] = 0;
-        b.rowk[(size_t)item * TILE_H + lane] = 0;
+        for (uint32_t j = lane; j < own_n; j += 64) b.keep[own_slot(j, own_q(j))] = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (!has[q]) continue;
+            if (lane == 0) b.tile_keep[tq[q]] = 0;
+            b.rowk[(size_t)tq[q] * TILE_H + lane] = 0;
+        }
         return;
     }
     const float margin = rad + 1.0f;
@@ -1927,27 +1950,32 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    for (uint32_t ib = 0; ib < cnt; ib += 64) {
-        const uint32_t i = ib + lane;
-        const bool own = i < cnt;
+    uint32_t kept[4] = {0, 0, 0, 0};
+    for (uint32_t ib = 0; ib < own_n; ib += 64) {
+        const uint32_t j = ib + lane;
+        const bool own = j < own_n;
+        const int q = own ? own_q(j) : 0;
         float m = 0.f, y = 0.f, x = 0.f;
         int32_t idx = -1;
+        size_t sl = 0;
         if (own) {
-            m = rmass[slot0 + i];
-            y = ry[slot0 + i];
-            x = rx[slot0 + i];
-            const uint32_t l = slots[slot0 + i];
-            idx = (ty * TILE_H + (int)(l / TILE_W)) * W + (tx * TILE_W + (int)(l % TILE_W));
+            sl = own_slot(j, q);
+            m = rmass[sl];
+            y = ry[sl];
+            x = rx[sl];
+            const uint32_t l = slots[sl];
+            idx = ((ty + (q >> 1)) * TILE_H + (int)(l / TILE_W)) * W +
+                  ((tx + (q & 1)) * TILE_W + (int)(l % TILE_W));
         }
-        const bool valid = own && ok(m, slot0 + i);
+        const bool valid = own && ok(m, sl);
         bool drop = false;
         // every lane tests its own candidate against the staged records in
         // turn (one broadcast 16-byte LDS read per record; the reads do not
         // depend on each other, so they pipeline)
         auto test_staged = [&](uint32_t n) {
 #pragma unroll 4
-            for (uint32_t j = 0; j < n; ++j) {
-                const float4 rr = s_rec[wave][j];
+            for (uint32_t k = 0; k < n; ++k) {
+                const float4 rr = s_rec[wave][k];
                 const float dy = rr.y - y, dx = rr.z - x;
                 float d = dy * dy;
                 d = d + dx * dx;
@@ -1964,8 +1992,8 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                     const int ng = min(64, ntl - g0);
                     uint32_t myc = 0, myit = 0;
                     if (lane < ng) {
-                        const int q = g0 + lane;
-                        const int ny = y_lo + q / nxr, nx = x_lo + q % nxr;
+                        const int qq = g0 + lane;
+                        const int ny = y_lo + qq / nxr, nx = x_lo + qq % nxr;
                         myit = fr.tile_base + (uint32_t)(ny * (int)fr.tx + nx);
                         myc = b.tile_count[myit];
                     }
@@ -1976,17 +2004,17 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                     do {
                         S = 0;
                         for (; jb < T; jb += 64) {
-                            const uint32_t j = jb + lane;
-                            // tile of record j: the last q whose first record is <= j
-                            int q = 0;
+                            const uint32_t jj = jb + lane;
+                            // tile of record jj: the last qq whose first record is <= jj
+                            int qt = 0;
                             for (int qq = 1; qq < ng; ++qq)  // wave-uniform: scalar read of lane qq
-                                if ((uint32_t)__builtin_amdgcn_readlane((int)exq, qq) <= j) q = qq;
-                            const uint32_t it2 = __shfl(myit, q, 64);
-                            const uint32_t i2 = j - __shfl(exq, q, 64);
+                                if ((uint32_t)__builtin_amdgcn_readlane((int)exq, qq) <= jj) qt = qq;
+                            const uint32_t it2 = __shfl(myit, qt, 64);
+                            const uint32_t i2 = jj - __shfl(exq, qt, 64);
                             const size_t s2 = (size_t)it2 * TILE_PX + i2;
                             bool take = false;
                             float nm = 0.f, yy = 0.f, xx = 0.f;
-                            if (j < T) {
+                            if (jj < T) {
                                 nm = rmass[s2];
                                 yy = ry[s2];
                                 xx = rx[s2];
@@ -1999,7 +2027,7 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
                             if (take) {
                                 const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
                                 const uint32_t l2 = slots[s2];
-                                const int qg = g0 + q;
+                                const int qg = g0 + qt;
                                 const int ny = y_lo + qg / nxr, nx = x_lo + qg % nxr;
                                 const int32_t ri = (ny * TILE_H + (int)(l2 / TILE_W)) * W +
                                                    (nx * TILE_W + (int)(l2 % TILE_W));
@@ -2017,21 +2045,27 @@ __global__ __launch_bounds__(256) void k_dedup(Batch b, const Geom* __restrict__
             }
         }
         const bool keep = valid && !drop;
-        if (own) b.keep[slot0 + i] = keep ? 1 : 0;
-        if (keep) atomicAdd(&s_kr[wave][(uint32_t)(idx / W) - (uint32_t)(ty * TILE_H)], 1u);
-        kept += (uint32_t)__popcll(__ballot(keep));
+        if (own) b.keep[sl] = keep ? 1 : 0;
+        if (keep) {
+            const int row = idx / W - (ty + (q >> 1)) * TILE_H;
+            atomicAdd(&s_kr[wave][q][row], 1u);
+        }
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) kept[qq] += (uint32_t)__popcll(__ballot(keep && q == qq));
     }
-    if (lane == 0) b.tile_keep[item] = kept;
-    // kept rows of this tile in pixel rows <= r (the emit order is raster)
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    b.rowk[(size_t)item * TILE_H + lane] = wave_incl_scan(s_kr[wave][lane], lane);
+    // kept rows of each tile in pixel rows <= r (the emit order is raster)
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (!has[q]) continue;
+        if (lane == 0) b.tile_keep[tq[q]] = kept[q];
+        b.rowk[(size_t)tq[q] * TILE_H + lane] = wave_incl_scan(s_kr[wave][q][lane], lane);
+    }
 }
 
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
     (void)g;
-    hipLaunchKernelGGL(k_dedup, dim3((b.tiles + 3) / 4), dim3(256), 0, s, b, d_geom);
+    hipLaunchKernelGGL(k_dedup, dim3((b.dblocks + 3) / 4), dim3(256), 0, s, b, d_geom);
     return hipGetLastError();
 }
 
