@@ -165,12 +165,15 @@ gt_status gt_context_set_launch_shape(gt_context* ctx, uint32_t seg_rows);
 
 /* Autotuner: times the batched path on n_frames synthetic uint16 frames of
  * h x w for every launch shape (rows per bandpass work item 64/128/256,
- * frames per launch automatic/16/32) and keeps the fastest in the context
- * for calls whose frames are all h x w with the same filter geometry. */
+ * frames per launch automatic/16/32, the centroid walk inside detect or in
+ * its own kernel where both are possible) and keeps the fastest in the
+ * context for calls whose frames are all h x w with the same filter
+ * geometry.  Results do not depend on the shape. */
 typedef struct gt_tuning {
     uint32_t seg_rows;            /* rows per bandpass work item */
     uint32_t frames_per_launch;   /* 0 = as many as the workspace holds */
     double device_ms_per_frame;   /* of the chosen shape */
+    uint32_t fused_walk;          /* 1: walk inside detect, 0: separate refine kernel */
 } gt_tuning;
 gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, uint32_t w,
                               uint32_t n_frames, gt_tuning* out);

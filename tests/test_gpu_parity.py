@@ -209,6 +209,7 @@ def test_autotune_keeps_results(frames512):
     ctx = _lib.Context(0)
     t = ctx.autotune(p, 512, 512, n_frames=8)
     assert t["seg_rows"] in (64, 128, 256) and t["frames_per_launch"] in (0, 16, 32)
+    assert t["fused_walk"] in (True, False)
     got = locate(frames512, p, ctx=ctx)
     assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
 

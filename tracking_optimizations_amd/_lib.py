@@ -117,7 +117,7 @@ class GtTiming(ctypes.Structure):
 
 class GtTuning(ctypes.Structure):
     _fields_ = [("seg_rows", ctypes.c_uint32), ("frames_per_launch", ctypes.c_uint32),
-                ("device_ms_per_frame", ctypes.c_double)]
+                ("device_ms_per_frame", ctypes.c_double), ("fused_walk", ctypes.c_uint32)]
 
 
 class GtLinkStats(ctypes.Structure):
@@ -314,7 +314,7 @@ class Context:
         check(load().gt_context_autotune(self.handle, ctypes.byref(params), h, w, n_frames,
                                          ctypes.byref(t)))
         return {"seg_rows": t.seg_rows, "frames_per_launch": t.frames_per_launch,
-                "device_ms_per_frame": t.device_ms_per_frame}
+                "device_ms_per_frame": t.device_ms_per_frame, "fused_walk": bool(t.fused_walk)}
 
     def synchronize(self) -> None:
         check(load().gt_synchronize(self.handle))

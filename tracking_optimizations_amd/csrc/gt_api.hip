@@ -402,7 +402,12 @@ struct gt_context {
     }
     // autotuned (rows per bandpass work item, frames per launch) per frame
     // shape and filter geometry
-    std::map<std::array<int64_t, 6>, std::pair<int, uint32_t>> tuned;
+    struct Tuned {
+        int seg;
+        uint32_t batch;
+        int fuse;
+    };
+    std::map<std::array<int64_t, 6>, Tuned> tuned;
     // per-stage event timing (forces one stream so that kernels do not overlap)
     int profiling = 0;
     std::vector<hipEvent_t> ev_pool;
@@ -805,8 +810,9 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         if (uniform) {
             auto it = ctx->tuned.find(tune_key(frames[0].h, frames[0].w, g));
             if (it != ctx->tuned.end()) {
-                seg = it->second.first;
-                batch_frames = it->second.second;
+                seg = it->second.seg;
+                batch_frames = it->second.batch;
+                if (!it->second.fuse) g.walk_in_detect = 0;
             }
         }
     }
@@ -1527,46 +1533,55 @@ gt_status gt_context_autotune(gt_context* ctx, const gt_params* p, uint32_t h, u
     const int saved_seg = ctx->seg_rows;
     const uint32_t saved_batch = ctx->batch_frames;
     const int saved_graphs = ctx->graphs;
+    const int saved_fuse = ctx->fuse_walk;
     ctx->graphs = 0;
     ctx->tuned.erase(tune_key(h, w, g));
     const int segs[] = {64, 128, 256};
     const uint32_t batches[] = {0, 16, 32};
+    // the walk inside detect or in its own kernel, where both are possible
+    const int nfuse = (saved_fuse && walk_in_detect_supported(g)) ? 2 : 1;
     double best = 1e30;
-    int best_seg = saved_seg;
+    int best_seg = saved_seg, best_fuse = saved_fuse;
     uint32_t best_batch = 0;
     gt_status st = GT_OK;
-    for (int sg : segs) {
-        if (g.has_pct && (int64_t)K1_STRIP * sg >= 65536) continue;
-        for (uint32_t bf : batches) {
-            if (bf && bf >= n_frames) continue;
-            ctx->seg_rows = sg;
-            ctx->batch_frames = bf;
-            double ms = 1e30;
-            for (int rep = 0; rep < 3 && st == GT_OK; ++rep) {
-                gt_result* r = nullptr;
-                st = gt_locate_u16(ctx, movie.as<uint16_t>(), n_frames, h, w, p,
-                                   GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE, &r);
-                gt_result_free(r);
-                if (rep > 0) ms = std::min(ms, (double)ctx->last.device_ms);
+    for (int fz = 0; fz < nfuse && st == GT_OK; ++fz) {
+        ctx->fuse_walk = fz == 0 ? saved_fuse : 0;
+        for (int sg : segs) {
+            if (g.has_pct && (int64_t)K1_STRIP * sg >= 65536) continue;
+            for (uint32_t bf : batches) {
+                if (bf && bf >= n_frames) continue;
+                ctx->seg_rows = sg;
+                ctx->batch_frames = bf;
+                double ms = 1e30;
+                for (int rep = 0; rep < 3 && st == GT_OK; ++rep) {
+                    gt_result* r = nullptr;
+                    st = gt_locate_u16(ctx, movie.as<uint16_t>(), n_frames, h, w, p,
+                                       GT_FRAMES_ON_DEVICE | GT_RESULT_ON_DEVICE, &r);
+                    gt_result_free(r);
+                    if (rep > 0) ms = std::min(ms, (double)ctx->last.device_ms);
+                }
+                if (st != GT_OK) break;
+                if (ms < best) {
+                    best = ms;
+                    best_seg = sg;
+                    best_batch = bf;
+                    best_fuse = ctx->fuse_walk;
+                }
             }
             if (st != GT_OK) break;
-            if (ms < best) {
-                best = ms;
-                best_seg = sg;
-                best_batch = bf;
-            }
         }
-        if (st != GT_OK) break;
     }
     ctx->seg_rows = saved_seg;
     ctx->batch_frames = saved_batch;
     ctx->graphs = saved_graphs;
+    ctx->fuse_walk = saved_fuse;
     if (st != GT_OK) return st;
-    ctx->tuned[tune_key(h, w, g)] = {best_seg, best_batch};
+    ctx->tuned[tune_key(h, w, g)] = {best_seg, best_batch, best_fuse};
     if (out) {
         out->seg_rows = (uint32_t)best_seg;
         out->frames_per_launch = best_batch;
         out->device_ms_per_frame = best / n_frames;
+        out->fused_walk = (uint32_t)(best_fuse && walk_in_detect_supported(g));
     }
     return GT_OK;
     });
