@@ -16,7 +16,10 @@ the CPU baseline is the oracle (the CPU restatement of the reference
 pipeline, compiled with -march=native on the host it runs on) on the first 64
 frames, on one thread and on the 16 threads of one GPU's CPU share.  The
 same line carries the single-frame latency of the 512x512 workload (plain GPU
-path and CPU oracle) and the uneven 4096-frame batch, batched and plain.
+path and CPU oracle), the uneven 4096-frame batch (batched and plain), the
+adjoint workload (256 x 512^2, localisation errors) and the locate + link
+workload (2000 x 1024^2, 3000 blinking particles, with the C oracle linker
+timed on the same rows).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -216,6 +219,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-batch figures of the default line")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the adjoint and locate + link figures of the default line")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-percentile", action="store_true",
@@ -423,6 +428,19 @@ def main() -> int:
         log(f"uneven batch (4096 frames): {uneven['frames_per_s']:.0f} frames/s batched, "
             f"{uneven['plain_frames_per_s']:.0f} plain, {uneven['batched_over_plain']:.2f}x")
 
+    # the adjoint (256 x 512^2) and locate + link (2000 x 1024^2) workloads,
+    # each with its own context and movie, summarised on the same line
+    adjoint = link = None
+    if not args.no_extra:
+        a = bench_adjoint(args, summary=True)
+        adjoint = {k: a[k] for k in ("value", "unit", "ms_per_step", "with_jacobian_frames_per_s",
+                                     "forward_only_frames_per_s")}
+        adjoint["workload"] = a["config"]["workload"]
+        lk = bench_link(args, summary=True)
+        link = {k: lk[k] for k in ("value", "unit", "ms_per_step", "locate_only_frames_per_s",
+                                   "link_rust_only_frames_per_s", "cpu_baseline")}
+        link["workload"] = lk["config"]["workload"]
+
     cpu = None
     c1 = None
     if not args.no_cpu and args.cpu_sample > 0:
@@ -479,6 +497,8 @@ def main() -> int:
         "plain": plain,
         "batched_over_plain": (fps / plain["frames_per_s"]) if plain else None,
         "uneven": uneven,
+        "adjoint": adjoint,
+        "locate_link": link,
         "c1_latency": c1,
         "cpu_baseline": cpu,
     }
@@ -548,7 +568,7 @@ def bench_uneven(args) -> int:
     return 0
 
 
-def bench_adjoint(args) -> int:
+def bench_adjoint(args, summary: bool = False):
     """Adjoint workload: 256 frames of 512x512 uint16 (300 spots, sigma 1.5,
     peaks U(200, 3000), background 100 + Poisson), diameter 9, separation 9,
     smoothing 9, percentile 64: forward rows plus per-particle Poisson errors
@@ -597,7 +617,7 @@ def bench_adjoint(args) -> int:
         _lib.Result(out)
     ctx.synchronize()
     fwd = nf * args.steps / (time.perf_counter() - t0)
-    print(json.dumps({"metric": "frames/s, locate + adjoint (Poisson localisation errors), "
+    line = {"metric": "frames/s, locate + adjoint (Poisson localisation errors), "
                                 "256 x 512^2 uint16",
                       "value": res["errors"][0], "unit": "frames/s", "n_gpus": 1,
                       "steps": args.steps, "warmup": args.warmup,
@@ -607,12 +627,15 @@ def bench_adjoint(args) -> int:
                       "config": {"workload": "adjoint 256 x 512x512, 300 spots/frame",
                                  "params": kw, "rows": res["errors"][1]},
                       "with_jacobian_frames_per_s": res["jacobian"][0],
-                      "forward_only_frames_per_s": fwd}), flush=True)
+                      "forward_only_frames_per_s": fwd}
     movie.free()
+    if summary:
+        return line
+    print(json.dumps(line), flush=True)
     return 0
 
 
-def bench_link(args) -> int:
+def bench_link(args, summary: bool = False):
     """Locate + link: 2000 frames of 1024x1024 uint16, 3000 Brownian particles
     (D = 0.5 px^2/frame, sigma 1.8, lognormal peaks with median 1200,
     background 120 + Poisson), each off in a frame with probability 0.05;
@@ -692,13 +715,14 @@ def bench_link(args) -> int:
                          f"linker: {same}"}
         log(f"CPU C oracle linker: {nf / dt:.0f} frames/s ({dt:.2f} s); ids identical: {same}")
         k = min(args.link_cpu_frames, nf)
-        sub = plain[plain[:, 0] < k]
-        t0 = time.perf_counter()
-        pref = link_rows(sub, 5.0, 3)
-        pdt = time.perf_counter() - t0
-        log(f"Python oracle linker: {k / pdt:.1f} frames/s on {k} frames; ids identical: "
-            f"{bool(np.array_equal(pref, ids[: len(pref)]))}")
-    print(json.dumps({"metric": "frames/s, locate + link (search_range 5, memory 3), "
+        if not summary and k > 0:
+            sub = plain[plain[:, 0] < k]
+            t0 = time.perf_counter()
+            pref = link_rows(sub, 5.0, 3)
+            pdt = time.perf_counter() - t0
+            log(f"Python oracle linker: {k / pdt:.1f} frames/s on {k} frames; ids identical: "
+                f"{bool(np.array_equal(pref, ids[: len(pref)]))}")
+    line = {"metric": "frames/s, locate + link (search_range 5, memory 3), "
                                 "2000 x 1024^2 uint16, 3000 particles",
                       "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
                       "warmup": args.warmup, "ms_per_step": 1000 * el / args.steps,
@@ -712,8 +736,11 @@ def bench_link(args) -> int:
                       "link": {k2: st[k2] for k2 in st},
                       "locate_only_frames_per_s": loc_fps,
                       "link_rust_only_frames_per_s": nf / link_el,
-                      "cpu_baseline": cpu}), flush=True)
+                      "cpu_baseline": cpu}
     movie.free()
+    if summary:
+        return line
+    print(json.dumps(line), flush=True)
     return 0
 
 
