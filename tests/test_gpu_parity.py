@@ -441,3 +441,34 @@ def test_graph_replay_host_chunks(ctx):
         print(f"[parity] graph replay of host chunks {rep}", flush=True)
         got = locate(host, p, ctx=g)
         assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]), rep
+
+
+@pytest.mark.parametrize("shift", [(0, 1), (3, -2), (17, 29)])
+def test_full_size_translation(ctx, shift):
+    """Two 2000x2000 windows of a 2048x2048 benchmark-movie frame, offset by
+    `shift`: rows whose candidate lies more than 40 px inside both windows
+    are the same rows, moved by the offset (candidate pixel exactly, mass bit
+    for bit, position to binary32 rounding) - a size-independent check of the
+    whole path at full size, without the oracle."""
+    dy, dx = shift
+    frame = synth.generate_host(ctx, synth.movie_2048(seed=2, n_frames=1))[0]
+    n = 2000
+    a = np.ascontiguousarray(frame[24:24 + n, 24:24 + n])
+    b = np.ascontiguousarray(frame[24 + dy:24 + dy + n, 24 + dx:24 + dx + n])
+    # the c2 filters and separation without its percentile: the threshold is a
+    # quantile of the whole frame and would differ between the windows
+    p = make_params(diameter=11, minmass=0, separation=11, smoothing_size=11)
+    ra, ca, _, _ = locate(a[None], p, ctx=ctx)
+    rb, cb, _, _ = locate(b[None], p, ctx=ctx)
+    m = 40
+    inside = lambda c, oy, ox: ((c // n - oy >= m) & (c // n - oy < n - m) &
+                                (c % n - ox >= m) & (c % n - ox < n - m))
+    ka = inside(ca, 0, 0) & inside(ca, dy, dx)
+    kb = inside(cb, -dy, -dx) & inside(cb, 0, 0)
+    print(f"[parity] translation {shift}: {ka.sum()} rows compared", flush=True)
+    assert ka.sum() > 10000
+    assert np.array_equal(ca[ka] - (dy * n + dx), cb[kb])
+    assert np.array_equal(ra[ka, 1], rb[kb, 1])
+    # positions below 2048 carry a binary32 ulp of 1.2e-4 in each window
+    assert np.abs(ra[ka, 2] - dy - rb[kb, 2]).max() <= 2.5e-4
+    assert np.abs(ra[ka, 3] - dx - rb[kb, 3]).max() <= 2.5e-4
