@@ -445,16 +445,17 @@ def test_graph_replay_host_chunks(ctx):
 
 @pytest.mark.parametrize("shift", [(0, 1), (3, -2), (17, 29)])
 def test_full_size_translation(ctx, shift):
-    """Two 2000x2000 windows of a 2048x2048 benchmark-movie frame, offset by
+    """Two 1984x1984 windows of a 2048x2048 benchmark-movie frame, offset by
     `shift`: rows whose candidate lies more than 40 px inside both windows
     are the same rows, moved by the offset (candidate pixel exactly, mass bit
     for bit, position to binary32 rounding) - a size-independent check of the
     whole path at full size, without the oracle."""
     dy, dx = shift
     frame = synth.generate_host(ctx, synth.movie_2048(seed=2, n_frames=1))[0]
-    n = 2000
+    n = 1984
     a = np.ascontiguousarray(frame[24:24 + n, 24:24 + n])
     b = np.ascontiguousarray(frame[24 + dy:24 + dy + n, 24 + dx:24 + dx + n])
+    assert a.shape == b.shape == (n, n)
     # the c2 filters and separation without its percentile: the threshold is a
     # quantile of the whole frame and would differ between the windows
     p = make_params(diameter=11, minmass=0, separation=11, smoothing_size=11)
