@@ -195,11 +195,14 @@ template <typename T> struct Chunk4;
 template <> struct Chunk4<uint16_t> { uint2 v; };
 template <> struct Chunk4<uint8_t> { uint32_t v; };
 
+// base: the frame (wave-uniform); off: byte offset, 32-bit (frames hold
+// fewer than 2^30 pixels), so the load takes the frame as a scalar base
 template <typename T>
-__device__ __forceinline__ Chunk4<T> ld_chunk(const GLOBAL T* row, int src) {
+__device__ __forceinline__ Chunk4<T> ld_chunk(const GLOBAL char* base, uint32_t off) {
+    const GLOBAL char* p = base + off;
     Chunk4<T> c;
-    if constexpr (sizeof(T) == 2) c.v = *(const GLOBAL uint2*)(row + src);
-    else c.v = *(const GLOBAL uint32_t*)(row + src);
+    if constexpr (sizeof(T) == 2) c.v = *(const GLOBAL uint2*)p;
+    else c.v = *(const GLOBAL uint32_t*)p;
     return c;
 }
 
@@ -291,6 +294,8 @@ void k_bandpass(Batch b, K1Params kp) {
     bool crev, coob;
     chunk_src(cc, W, csrc, crev, coob);
     const bool outl = t >= K1_LANE0 && t < K1_LANE0 + K1_LANES && cc < W;
+    const uint32_t cboff = (uint32_t)csrc * (uint32_t)sizeof(T);
+    const uint32_t pcoff = 4u * (uint32_t)cc;   // used by output lanes only
     // first staged column this lane reads (halo lanes read in range, unused)
     const int rbase = min(max(4 * t - 8, 0), K1_ROWBUF - 20);
 
@@ -310,9 +315,11 @@ void k_bandpass(Batch b, K1Params kp) {
     float vbox[4] = {0.f, 0.f, 0.f, 0.f};
 
     const int vr0 = y0 - R, vr1 = y1 + R;
-    auto row_ptr = [&](int vr) {
+    const GLOBAL char* inb = (const GLOBAL char*)in;
+    // byte offset of row vr, reflected at the top and bottom edges
+    auto row_off = [&](int vr) -> uint32_t {
         const int ar = vr < 0 ? -vr - 1 : (vr >= H ? 2 * H - vr - 1 : vr);
-        return in + (size_t)ar * (size_t)W;
+        return (uint32_t)ar * (uint32_t)W * (uint32_t)sizeof(T);
     };
 
     // prefetch: row vr is loaded PF rows ahead into slot (vr - vr0) % NT,
@@ -321,7 +328,7 @@ void k_bandpass(Batch b, K1Params kp) {
     Chunk4<T> pm[NT];
 #pragma unroll
     for (int s = 0; s < PF; ++s) {
-        if (vr0 + s < vr1) pm[s] = ld_chunk<T>(row_ptr(vr0 + s), csrc);
+        if (vr0 + s < vr1) pm[s] = ld_chunk<T>(inb, row_off(vr0 + s) + cboff);
     }
 
     uint32_t zc = 0;  // zero outputs of this work item (wave-uniform)
@@ -356,7 +363,7 @@ void k_bandpass(Batch b, K1Params kp) {
                         *(f4v*)(bg_ + 4 * t) = m;
                     }
                 }
-                if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(row_ptr(vr + PF), csrc);
+                if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(inb, row_off(vr + PF) + cboff);
                 __syncthreads();
 
                 // boxcar row pass on the mirrored copy (integer sums: exact)
@@ -408,19 +415,26 @@ void k_bandpass(Batch b, K1Params kp) {
                         out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
                     }
                     if (outl) {
-                        *(GLOBAL float4*)(P + (size_t)o * (size_t)W + cc) =
+                        // the row offset goes through a scalar register, so
+                        // each store adds one uniform value to this lane's
+                        // column offset
+                        const uint32_t ro = (uint32_t)__builtin_amdgcn_readfirstlane(
+                            (int)(4u * (uint32_t)o * (uint32_t)W));
+                        *(GLOBAL float4*)((GLOBAL char*)P + (ro + pcoff)) =
                             make_float4(out[0], out[1], out[2], out[3]);
                     }
-                    if (kp.hist) {  // wave-uniform: the zero count is a ballot over all lanes
+                    if (kp.hist) {  // wave-uniform
                         // bin (bits >> 19) + 1 for positive values (finite and
                         // >= +0, so no clamp is needed): word bits >> 20, half
                         // bit 19; zero pixels (most of a frame) are counted by
-                        // a ballot instead of contending for one word
+                        // a ballot over the output lanes (one compare serves
+                        // both) instead of contending for one word
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             const uint32_t bits = __float_as_uint(out[q]);
-                            zc += (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(bits == 0u) & outl_m);
-                            if (outl && bits != 0u)
+                            const bool pos = bits != 0u;
+                            zc += (uint32_t)__popcll(~__builtin_amdgcn_ballot_w64(pos) & outl_m);
+                            if (outl && pos)
                                 atomicAdd(&s_hist[bits >> 20], 1u << ((bits >> 15) & 16u));
                         }
                     }
