@@ -1872,6 +1872,13 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
     // staged records, one 16-byte entry each: mass, y, x, raster index (bits)
     __shared__ float4 s_rec[4][DD_CAP];
     __shared__ uint32_t s_kr[4][4][TILE_H];   // kept rows per block tile and pixel row
+    // staged records are ordered by 64 horizontal bands: s_perm lists them
+    // band by band, band b starting at s_band[wave][b] (s_band[wave][64] =
+    // staged count)
+    __shared__ uint32_t s_band[4][65];
+    __shared__ uint32_t s_bcur[4][64];   // fill cursors of the bands
+    __shared__ uint8_t s_perm[4][DD_CAP];
+    static_assert(DD_CAP <= 256, "record numbers are bytes");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t item = blockIdx.x * 4 + wave;
     if (item >= b.dblocks) return;
@@ -1947,8 +1954,19 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
         }
         return;
     }
+    // wave-uniform from here on: keep them in scalar registers
+    auto uni = [](float v) {
+        return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+    };
     const float margin = rad + 1.0f;
-    bymin -= margin; bymax += margin; bxmin -= margin; bxmax += margin;
+    bymin = uni(bymin - margin); bymax = uni(bymax + margin);
+    bxmin = uni(bxmin - margin); bxmax = uni(bxmax + margin);
+    // bands at least rad + 1 high: a record strictly within `separation` of
+    // a position lies in that position's band or in one of its two
+    // neighbours (band_of is monotone; its rounding is far below one band)
+    const float bh = fmaxf(rad + 1.0f, (bymax - bymin) * (1.0f / 63.0f));
+    const float binv = uni(1.0f / bh);
+    auto band_of = [&](float yy) { return min(63, max(0, (int)((yy - bymin) * binv))); };
 
     // The neighbourhood is staged in rounds of at most DD_CAP records: the
     // neighbour tiles are taken 64 at a time (lane q holds tile q's count),
@@ -1983,23 +2001,49 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
         }
         const bool valid = own && ok(m, sl);
         bool drop = false;
-        // every lane tests its own candidate against the staged records in
-        // turn (one broadcast 16-byte LDS read per record; the reads do not
-        // depend on each other, so they pipeline)
-        auto test_staged = [&](uint32_t n) {
-#pragma unroll 4
-            for (uint32_t k = 0; k < n; ++k) {
-                const float4 rr = s_rec[wave][k];
+        // Counting sort of the n staged records by band into s_perm: count,
+        // scan, then place through per-band cursors (the band is recomputed
+        // from the staged y, so nothing is held across the passes).
+        auto band_sort = [&](uint32_t n) {
+            s_band[wave][lane] = 0;
+            wave_sync();
+            for (uint32_t k = lane; k < n; k += 64)
+                atomicAdd(&s_band[wave][band_of(s_rec[wave][k].y)], 1u);
+            wave_sync();
+            const uint32_t c = s_band[wave][lane];
+            const uint32_t inc = wave_incl_scan(c, lane);
+            s_band[wave][lane] = inc - c;
+            s_bcur[wave][lane] = inc - c;
+            if (lane == 63) s_band[wave][64] = inc;
+            wave_sync();
+            for (uint32_t k = lane; k < n; k += 64) {
+                const uint32_t pos = atomicAdd(&s_bcur[wave][band_of(s_rec[wave][k].y)], 1u);
+                s_perm[wave][pos] = (uint8_t)k;
+            }
+            wave_sync();
+        };
+        // every lane tests its own candidate against the records of its band
+        // and the two neighbouring bands (execute_gpu.rs:258-284).  A lane's
+        // own record never drops it (equal mass, equal raster index), and
+        // distinct candidates have distinct raster indices, so no self test
+        // is needed.
+        auto test_banded = [&]() {
+            if (!valid) return;
+            const int b0 = band_of(y);
+            const uint32_t lo = s_band[wave][max(b0 - 1, 0)];
+            const uint32_t hi = s_band[wave][min(b0 + 2, 64)];
+            for (uint32_t k = lo; k < hi; ++k) {
+                const float4 rr = s_rec[wave][s_perm[wave][k]];
                 const float dy = rr.y - y, dx = rr.z - x;
                 float d = dy * dy;
                 d = d + dx * dx;
                 const int32_t nidx = __float_as_int(rr.w);
-                drop |= d < r2 && nidx != idx && (m < rr.x || (m == rr.x && idx > nidx));
+                drop |= d < r2 && (m < rr.x || (m == rr.x && idx > nidx));
             }
         };
         if (__ballot(valid)) {  // wave-uniform
             if (resident) {
-                test_staged(S);
+                test_banded();
             } else {
                 bool single = true;   // one round held the whole neighbourhood
                 for (int g0 = 0; g0 < ntl; g0 += 64) {
@@ -2050,7 +2094,8 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
                             S += nt;
                         }
                         wave_sync();
-                        test_staged(S);
+                        band_sort(S);
+                        test_banded();
                         if (jb < T || g0 + 64 < ntl) single = false;
                         wave_sync();   // every lane's reads done before the next round's writes
                     } while (jb < T);
