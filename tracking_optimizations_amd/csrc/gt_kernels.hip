@@ -1866,17 +1866,19 @@ hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom, hipS
 // count and kept rows per pixel row.
 // ---------------------------------------------------------------------------
 constexpr int DD_CAP = 256;   // staged records per wave and round
+constexpr int DD_CELLS_1D = 16;
+constexpr int DD_CELLS = DD_CELLS_1D * DD_CELLS_1D;   // cells of a staged neighbourhood
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 void k_dedup(Batch b, const Geom* __restrict__ g) {
     // staged records, one 16-byte entry each: mass, y, x, raster index (bits)
     __shared__ float4 s_rec[4][DD_CAP];
     __shared__ uint32_t s_kr[4][4][TILE_H];   // kept rows per block tile and pixel row
-    // staged records are ordered by 64 horizontal bands: s_perm lists them
-    // band by band, band b starting at s_band[wave][b] (s_band[wave][64] =
-    // staged count)
-    __shared__ uint32_t s_band[4][65];
-    __shared__ uint32_t s_bcur[4][64];   // fill cursors of the bands
+    // staged records are ordered by square cells (row-major, at most 16 x
+    // 16): s_perm lists them cell by cell, cell c starting at s_cell[wave][c]
+    // (s_cell[wave][DD_CELLS] = staged count)
+    __shared__ uint16_t s_cell[4][DD_CELLS + 1];
+    __shared__ uint32_t s_ccur[4][DD_CELLS / 2];   // 16-bit counts, then fill cursors
     __shared__ uint8_t s_perm[4][DD_CAP];
     static_assert(DD_CAP <= 256, "record numbers are bytes");
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1961,12 +1963,15 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
     const float margin = rad + 1.0f;
     bymin = uni(bymin - margin); bymax = uni(bymax + margin);
     bxmin = uni(bxmin - margin); bxmax = uni(bxmax + margin);
-    // bands at least rad + 1 high: a record strictly within `separation` of
-    // a position lies in that position's band or in one of its two
-    // neighbours (band_of is monotone; its rounding is far below one band)
-    const float bh = fmaxf(rad + 1.0f, (bymax - bymin) * (1.0f / 63.0f));
-    const float binv = uni(1.0f / bh);
-    auto band_of = [&](float yy) { return min(63, max(0, (int)((yy - bymin) * binv))); };
+    // cells at least rad + 1 wide: a record strictly within `separation` of
+    // a position lies in that position's cell or one of its eight neighbours
+    // (cell coordinates are monotone; their rounding is far below one cell)
+    const float ch = fmaxf(rad + 1.0f, fmaxf(bymax - bymin, bxmax - bxmin) * (1.0f / 15.0f));
+    const float cinv = uni(1.0f / ch);
+    const int ncy = min(DD_CELLS_1D, (int)((bymax - bymin) * cinv) + 1);
+    const int ncx = min(DD_CELLS_1D, (int)((bxmax - bxmin) * cinv) + 1);
+    auto cell_y = [&](float yy) { return min(ncy - 1, max(0, (int)((yy - bymin) * cinv))); };
+    auto cell_x = [&](float xx) { return min(ncx - 1, max(0, (int)((xx - bxmin) * cinv))); };
 
     // The neighbourhood is staged in rounds of at most DD_CAP records: the
     // neighbour tiles are taken 64 at a time (lane q holds tile q's count),
@@ -2001,49 +2006,75 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
         }
         const bool valid = own && ok(m, sl);
         bool drop = false;
-        // Counting sort of the n staged records by band into s_perm: count,
-        // scan, then place through per-band cursors (the band is recomputed
-        // from the staged y, so nothing is held across the passes).
-        auto band_sort = [&](uint32_t n) {
-            s_band[wave][lane] = 0;
+        // Counting sort of the n staged records by cell into s_perm: count,
+        // scan (lane l owns cells 4l .. 4l + 3), then place through per-cell
+        // cursors (the cell is recomputed from the staged position, so
+        // nothing is held across the passes).
+        auto cell_of = [&](uint32_t k) {
+            const float4 rr = s_rec[wave][k];
+            return cell_y(rr.y) * ncx + cell_x(rr.z);
+        };
+        // Counts and cursors are 16-bit halves of 32-bit words (at most
+        // DD_CAP records).
+        auto cell_add = [&](int c) {
+            const uint32_t sh = 16u * (uint32_t)(c & 1);
+            return (atomicAdd(&s_ccur[wave][c >> 1], 1u << sh) >> sh) & 0xffffu;
+        };
+        auto cell_sort = [&](uint32_t n) {
+            constexpr int PC = DD_CELLS / 64;   // cells per lane in the scan
+            static_assert(PC % 2 == 0, "a lane's cells fill whole words");
+#pragma unroll
+            for (int i = 0; i < PC / 2; ++i) s_ccur[wave][(PC / 2) * lane + i] = 0;
             wave_sync();
-            for (uint32_t k = lane; k < n; k += 64)
-                atomicAdd(&s_band[wave][band_of(s_rec[wave][k].y)], 1u);
+            for (uint32_t k = lane; k < n; k += 64) cell_add(cell_of(k));
             wave_sync();
-            const uint32_t c = s_band[wave][lane];
-            const uint32_t inc = wave_incl_scan(c, lane);
-            s_band[wave][lane] = inc - c;
-            s_bcur[wave][lane] = inc - c;
-            if (lane == 63) s_band[wave][64] = inc;
-            wave_sync();
-            for (uint32_t k = lane; k < n; k += 64) {
-                const uint32_t pos = atomicAdd(&s_bcur[wave][band_of(s_rec[wave][k].y)], 1u);
-                s_perm[wave][pos] = (uint8_t)k;
+            uint32_t c[PC], tot = 0;
+#pragma unroll
+            for (int i = 0; i < PC / 2; ++i) {
+                const uint32_t w2 = s_ccur[wave][(PC / 2) * lane + i];
+                c[2 * i] = w2 & 0xffffu;
+                c[2 * i + 1] = w2 >> 16;
+                tot += c[2 * i] + c[2 * i + 1];
             }
+            uint32_t run = wave_incl_scan(tot, lane) - tot;
+#pragma unroll
+            for (int i = 0; i < PC / 2; ++i) {
+                const uint32_t r0 = run, r1 = run + c[2 * i];
+                s_cell[wave][PC * lane + 2 * i] = (uint16_t)r0;
+                s_cell[wave][PC * lane + 2 * i + 1] = (uint16_t)r1;
+                s_ccur[wave][(PC / 2) * lane + i] = r0 | (r1 << 16);
+                run = r1 + c[2 * i + 1];
+            }
+            if (lane == 63) s_cell[wave][DD_CELLS] = (uint16_t)run;
+            wave_sync();
+            for (uint32_t k = lane; k < n; k += 64) s_perm[wave][cell_add(cell_of(k))] = (uint8_t)k;
             wave_sync();
         };
-        // every lane tests its own candidate against the records of its band
-        // and the two neighbouring bands (execute_gpu.rs:258-284).  A lane's
-        // own record never drops it (equal mass, equal raster index), and
-        // distinct candidates have distinct raster indices, so no self test
-        // is needed.
-        auto test_banded = [&]() {
+        // every lane tests its own candidate against the records of its cell
+        // and the eight neighbouring cells, one contiguous run of cells per
+        // cell row (execute_gpu.rs:258-284).  A lane's own record never
+        // drops it (equal mass, equal raster index), and distinct candidates
+        // have distinct raster indices, so no self test is needed.
+        auto test_cells = [&]() {
             if (!valid) return;
-            const int b0 = band_of(y);
-            const uint32_t lo = s_band[wave][max(b0 - 1, 0)];
-            const uint32_t hi = s_band[wave][min(b0 + 2, 64)];
-            for (uint32_t k = lo; k < hi; ++k) {
-                const float4 rr = s_rec[wave][s_perm[wave][k]];
-                const float dy = rr.y - y, dx = rr.z - x;
-                float d = dy * dy;
-                d = d + dx * dx;
-                const int32_t nidx = __float_as_int(rr.w);
-                drop |= d < r2 && (m < rr.x || (m == rr.x && idx > nidx));
+            const int cy0 = cell_y(y), cx0 = cell_x(x);
+            const int cxa = max(cx0 - 1, 0), cxb = min(cx0 + 1, ncx - 1);
+            for (int cy = max(cy0 - 1, 0); cy <= min(cy0 + 1, ncy - 1); ++cy) {
+                const uint32_t lo = s_cell[wave][cy * ncx + cxa];
+                const uint32_t hi = s_cell[wave][cy * ncx + cxb + 1];
+                for (uint32_t k = lo; k < hi; ++k) {
+                    const float4 rr = s_rec[wave][s_perm[wave][k]];
+                    const float dy = rr.y - y, dx = rr.z - x;
+                    float d = dy * dy;
+                    d = d + dx * dx;
+                    const int32_t nidx = __float_as_int(rr.w);
+                    drop |= d < r2 && (m < rr.x || (m == rr.x && idx > nidx));
+                }
             }
         };
         if (__ballot(valid)) {  // wave-uniform
             if (resident) {
-                test_banded();
+                test_cells();
             } else {
                 bool single = true;   // one round held the whole neighbourhood
                 for (int g0 = 0; g0 < ntl; g0 += 64) {
@@ -2094,8 +2125,8 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
                             S += nt;
                         }
                         wave_sync();
-                        band_sort(S);
-                        test_banded();
+                        cell_sort(S);
+                        test_cells();
                         if (jb < T || g0 + 64 < ntl) single = false;
                         wave_sync();   // every lane's reads done before the next round's writes
                     } while (jb < T);
