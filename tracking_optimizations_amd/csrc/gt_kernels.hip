@@ -1922,8 +1922,11 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
     const bool use_thr = g->walk_in_detect && g->has_pct;
     const double thr = use_thr ? b.thr[f] : 0.0;
     const GLOBAL float* rcen = (const GLOBAL float*)(b.rec + REC_CENTER * cap);
-    auto ok = [&](float mass, size_t s) {
-        return mass != 0.f && (!use_thr || (double)rcen[s] > thr);
+    // a record's start value (percentile test), loaded beside its other
+    // fields so that a record costs one round trip to memory
+    auto cen_of = [&](size_t s) { return use_thr ? rcen[s] : 0.f; };
+    auto ok = [&](float mass, float cen) {
+        return (mass != 0.f) & (!use_thr | ((double)cen > thr));
     };
     const int y_lo = max(0, ty - rty), y_hi = min((int)fr.ty - 1, ty + 1 + rty);
     const int x_lo = max(0, tx - rtx), x_hi = min((int)fr.tx - 1, tx + 1 + rtx);
@@ -1934,8 +1937,8 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
     float bymin = INFINITY, bymax = -INFINITY, bxmin = INFINITY, bxmax = -INFINITY;
     for (uint32_t j = lane; j < own_n; j += 64) {
         const size_t sl = own_slot(j, own_q(j));
-        if (ok(rmass[sl], sl)) {
-            const float y = ry[sl], x = rx[sl];
+        const float m0 = rmass[sl], c0 = cen_of(sl), y = ry[sl], x = rx[sl];
+        if (ok(m0, c0)) {
             bymin = fminf(bymin, y); bymax = fmaxf(bymax, y);
             bxmin = fminf(bxmin, x); bxmax = fmaxf(bxmax, x);
         }
@@ -1995,16 +1998,18 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
         float m = 0.f, y = 0.f, x = 0.f;
         int32_t idx = -1;
         size_t sl = 0;
+        float cen = 0.f;
         if (own) {
             sl = own_slot(j, q);
             m = rmass[sl];
             y = ry[sl];
             x = rx[sl];
+            cen = cen_of(sl);
             const uint32_t l = slots[sl];
             idx = ((ty + (q >> 1)) * TILE_H + (int)(l / TILE_W)) * W +
                   ((tx + (q & 1)) * TILE_W + (int)(l % TILE_W));
         }
-        const bool valid = own && ok(m, sl);
+        const bool valid = own && ok(m, cen);
         bool drop = false;
         // Counting sort of the n staged records by cell into s_perm: count,
         // scan (lane l owns cells 4l .. 4l + 3), then place through per-cell
@@ -2102,20 +2107,22 @@ void k_dedup(Batch b, const Geom* __restrict__ g) {
                             const uint32_t i2 = jj - __shfl(exq, qt, 64);
                             const size_t s2 = (size_t)it2 * TILE_PX + i2;
                             bool take = false;
-                            float nm = 0.f, yy = 0.f, xx = 0.f;
-                            if (jj < T) {
+                            float nm = 0.f, yy = 0.f, xx = 0.f, cn = 0.f;
+                            uint32_t l2 = 0;
+                            if (jj < T) {   // every field of the record in one round trip
                                 nm = rmass[s2];
                                 yy = ry[s2];
                                 xx = rx[s2];
-                                take = ok(nm, s2) && yy >= bymin && yy <= bymax && xx >= bxmin &&
-                                       xx <= bxmax;
+                                cn = cen_of(s2);
+                                l2 = slots[s2];
+                                take = ok(nm, cn) & (yy >= bymin) & (yy <= bymax) & (xx >= bxmin) &
+                                       (xx <= bxmax);
                             }
                             const uint64_t tm = __ballot(take);
                             const uint32_t nt = (uint32_t)__popcll(tm);
                             if (S + nt > (uint32_t)DD_CAP) break;   // test, then resume at jb
                             if (take) {
                                 const uint32_t r = S + (uint32_t)__popcll(tm & lanemask_lt(lane));
-                                const uint32_t l2 = slots[s2];
                                 const int qg = g0 + qt;
                                 const int ny = y_lo + qg / nxr, nx = x_lo + qg % nxr;
                                 const int32_t ri = (ny * TILE_H + (int)(l2 / TILE_W)) * W +
