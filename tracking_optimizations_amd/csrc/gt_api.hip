@@ -931,17 +931,30 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
         rows_cap = std::min<uint64_t>(ctx->rows.bytes / (4 * (uint64_t)g.ncols), ctx->rows_cand.bytes / 8);
     }
 
-    // ---- frame tables for every batch (uploaded once) ---------------------
+    // ---- frame tables -------------------------------------------------------
+    // The host builds the frame table and, per batch, the small base arrays
+    // (first bandpass item, first tile and first dedup block of every frame:
+    // 3 x (nb + 1) entries); k_fill_tables expands them on the device into the
+    // per-item tables (frame of every bandpass item and tile, top-left tile of
+    // every dedup block), so a call uploads kilobytes, not megabytes.
     std::vector<Frame> ftab(frames.size());
-    std::vector<uint32_t> bases_all;  // per batch: k1 bases [nb+1] then tile bases [nb+1]
-    std::vector<size_t> bases_off(plan.size());
-    std::vector<uint32_t> dblocks_of(plan.size());
+    std::vector<uint32_t> small;              // per batch: k1, tile and dedup-block bases
+    std::vector<size_t> small_off(plan.size());
+    std::vector<uint64_t> big_off(plan.size());   // per batch: k1_frame, tile_frame, dblock_tile
+    std::vector<uint32_t> k1_items_of(plan.size()), tiles_of(plan.size()), dblocks_of(plan.size());
+    uint32_t max_frame_work = 1;              // largest bandpass items + tiles + blocks of a frame
+    uint64_t big_total = 0;
     for (size_t bi = 0; bi < plan.size(); ++bi) {
         const auto& bp = plan[bi];
         Workspace& wsb = ctx->ws[bi % nstreams];
+        const size_t nb = bp.b1 - bp.b0;
         uint64_t pxo = 0, inoff = 0;
-        uint32_t k1b = 0, tb = 0;
-        std::vector<uint32_t> k1v, tbv;
+        uint32_t k1b = 0, tb = 0, db = 0;
+        small_off[bi] = small.size();
+        small.resize(small.size() + 3 * (nb + 1));
+        uint32_t* k1v = small.data() + small_off[bi];
+        uint32_t* tbv = k1v + nb + 1;
+        uint32_t* dbv = tbv + nb + 1;
         for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
             const FrameSizes fs = sizes_of(frames[fi].h, frames[fi].w, seg);
             Frame& F = ftab[fi];
@@ -964,35 +977,29 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             F.k1_base = k1b;
             F.slot_base = (uint64_t)tb * TILE_PX;
             F.frame_index = frames[fi].fi;
-            k1v.push_back(k1b);
-            tbv.push_back(tb);
+            const uint32_t nblk = ((fs.ty + 1) / 2) * ((fs.tx + 1) / 2);
+            k1v[fi - bp.b0] = k1b;
+            tbv[fi - bp.b0] = tb;
+            dbv[fi - bp.b0] = db;
+            max_frame_work = std::max<uint32_t>(max_frame_work, fs.k1_items + fs.tiles + nblk);
             k1b += fs.k1_items;
             tb += fs.tiles;
+            db += nblk;
             pxo += fs.px;
         }
-        k1v.push_back(k1b);
-        tbv.push_back(tb);
-        bases_off[bi] = bases_all.size();
-        bases_all.insert(bases_all.end(), k1v.begin(), k1v.end());
-        bases_all.insert(bases_all.end(), tbv.begin(), tbv.end());
-        // frame of every bandpass item and every detect tile (batch-relative)
-        for (uint32_t fi = 0; fi + 1 < (uint32_t)k1v.size(); ++fi)
-            for (uint32_t i = k1v[fi]; i < k1v[fi + 1]; ++i) bases_all.push_back(fi);
-        for (uint32_t fi = 0; fi + 1 < (uint32_t)tbv.size(); ++fi)
-            for (uint32_t i = tbv[fi]; i < tbv[fi + 1]; ++i) bases_all.push_back(fi);
-        // top-left tile of every 2x2 block of tiles (dedup)
-        const size_t d0 = bases_all.size();
-        for (size_t fi = bp.b0; fi < bp.b1; ++fi) {
-            const Frame& F = ftab[fi];
-            for (uint32_t ty = 0; ty < F.ty; ty += 2)
-                for (uint32_t tx = 0; tx < F.tx; tx += 2) bases_all.push_back(F.tile_base + ty * F.tx + tx);
-        }
-        dblocks_of[bi] = (uint32_t)(bases_all.size() - d0);
+        k1v[nb] = k1b;
+        tbv[nb] = tb;
+        dbv[nb] = db;
+        k1_items_of[bi] = k1b;
+        tiles_of[bi] = tb;
+        dblocks_of[bi] = db;
+        big_off[bi] = big_total;
+        big_total += (uint64_t)k1b + tb + db;
     }
     HIP_TRY(ctx->frames_tab.ensure(ftab.size() * sizeof(Frame)));
-    HIP_TRY(ctx->bases.ensure(bases_all.size() * 4));
+    HIP_TRY(ctx->bases.ensure((small.size() + big_total) * 4));
     HIP_TRY(hipMemcpy(ctx->frames_tab.p, ftab.data(), ftab.size() * sizeof(Frame), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ctx->bases.p, bases_all.data(), bases_all.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->bases.p, small.data(), small.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->geom.p, &g, sizeof(Geom), hipMemcpyHostToDevice));
     const Geom* d_geom = ctx->geom.as<const Geom>();
 
@@ -1037,12 +1044,13 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
             Batch b;
             std::memset(&b, 0, sizeof(b));
             b.frames = ctx->frames_tab.as<const Frame>() + bp.b0;
-            b.k1_bases = ctx->bases.as<const uint32_t>() + bases_off[bi];
+            b.k1_bases = ctx->bases.as<const uint32_t>() + small_off[bi];
             b.tile_bases = b.k1_bases + nb + 1;
             b.nframes = nb;
-            b.k1_items = bases_all[bases_off[bi] + nb];
-            b.tiles = bases_all[bases_off[bi] + nb + 1 + nb];
-            b.k1_frame = b.tile_bases + nb + 1;
+            b.k1_items = k1_items_of[bi];
+            b.tiles = tiles_of[bi];
+            const uint32_t* dblock_bases = b.tile_bases + nb + 1;
+            b.k1_frame = ctx->bases.as<const uint32_t>() + small.size() + big_off[bi];
             b.tile_frame = b.k1_frame + b.k1_items;
             b.dblock_tile = b.tile_frame + b.tiles;
             b.dblocks = dblocks_of[bi];
@@ -1085,8 +1093,11 @@ gt_status run_locate(gt_context* ctx, const std::vector<FrameIn>& frames, int in
                     off += al256(nbytes);
                 }
             }
-            HIP_TRY(stage(ctx, st, GT_STAGE_ZERO, [&] { return launch_zero_batch(b, st); }));
-            ++launches;
+            HIP_TRY(stage(ctx, st, GT_STAGE_ZERO, [&] {
+                const hipError_t e = launch_fill_tables(b, dblock_bases, max_frame_work, st);
+                return e != hipSuccess ? e : launch_zero_batch(b, st);
+            }));
+            launches += 2;
             if (generic) {
                 uint32_t maxpx = 0;
                 for (size_t fi = bp.b0; fi < bp.b1; ++fi)

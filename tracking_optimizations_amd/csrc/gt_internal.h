@@ -156,6 +156,10 @@ bool detect_needs_plane(const Geom& g);
 hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_select2(const Batch& b, const Geom& g, hipStream_t s);
+// Expands the batch's bases (first bandpass item, tile and dedup block of
+// every frame) into b.k1_frame, b.tile_frame and b.dblock_tile.
+hipError_t launch_fill_tables(const Batch& b, const uint32_t* dblock_bases,
+                              uint32_t max_frame_work, hipStream_t s);
 hipError_t launch_refine(const Batch& b, const Geom& g, const Geom* d_geom,
                          hipStream_t s);
 hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom,

@@ -118,6 +118,40 @@ hipError_t launch_zero_batch(const Batch& b, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Per-item tables of a batch from its per-frame bases: frame blockIdx.y
+// writes the frame index of each of its bandpass items and tiles and the
+// top-left tile of each of its 2x2 dedup blocks (the frame's blocks in
+// raster order).
+__global__ void k_fill_tables(Batch b, const uint32_t* __restrict__ dbases) {
+    const uint32_t f = blockIdx.y;
+    const uint32_t k1a = b.k1_bases[f], k1n = b.k1_bases[f + 1] - k1a;
+    const uint32_t ta = b.tile_bases[f], tn = b.tile_bases[f + 1] - ta;
+    const uint32_t da = dbases[f], dn = dbases[f + 1] - da;
+    const uint32_t tx = b.frames[f].tx;
+    const uint32_t bx = (tx + 1) / 2;   // dedup blocks per block row
+    uint32_t* k1f = (uint32_t*)b.k1_frame;
+    uint32_t* tf = (uint32_t*)b.tile_frame;
+    uint32_t* dt = (uint32_t*)b.dblock_tile;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k1n + tn + dn;
+         i += gridDim.x * blockDim.x) {
+        if (i < k1n) {
+            k1f[k1a + i] = f;
+        } else if (i < k1n + tn) {
+            tf[ta + (i - k1n)] = f;
+        } else {
+            const uint32_t j = i - k1n - tn;
+            dt[da + j] = ta + 2 * (j / bx) * tx + 2 * (j % bx);
+        }
+    }
+}
+
+hipError_t launch_fill_tables(const Batch& b, const uint32_t* dblock_bases,
+                              uint32_t max_frame_work, hipStream_t s) {
+    const uint32_t bx = std::min<uint32_t>((max_frame_work + 255) / 256, 64);
+    hipLaunchKernelGGL(k_fill_tables, dim3(bx, b.nframes), dim3(256), 0, s, b, dblock_bases);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
