@@ -1848,7 +1848,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         auto take = [&](size_t bytes) { const size_t at = o; o += al256(bytes); return at; };
         L.tiles = nb * fs.tiles;
         L.k1 = nb * fs.k1_items;
-        L.nbases = 2 * (nb + 1) + L.k1 + L.tiles + nb * dblocks_pf;
+        L.nbases = 3 * (nb + 1) + L.k1 + L.tiles + nb * dblocks_pf;
         const uint64_t slots = L.tiles * TILE_PX;
         L.geom = take(sizeof(Geom));
         L.counters = take(16);
@@ -1869,7 +1869,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
     auto fits = [&](uint64_t nb) {
         const Layout L = layout(nb);
         return L.end <= (size_t)GT_NOALLOC_SCRATCH_BYTES && nb <= gt_context::kNaFrames &&
-               L.nbases <= gt_context::kNaBases;
+               3 * (nb + 1) <= gt_context::kNaBases;
     };
     if (n == 0) return GT_OK;
     if (!fits(1))
@@ -1898,13 +1898,12 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
     for (uint64_t b0 = 0; b0 < n; b0 += nbmax) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(nbmax, n - b0);
         if (b0 > 0) HIP_TRY(hipEventSynchronize(ctx->ev_na));   // host tables copied out
-        // frame tables: k1 bases [nb + 1], tile bases [nb + 1], frame of every
-        // bandpass item, frame of every detect tile (batch-relative)
+        // frame tables: k1 bases [nb + 1], tile bases [nb + 1] and dedup-block
+        // bases [nb + 1] from the host; the per-item tables after them are
+        // filled on the device (k_fill_tables)
         uint32_t* k1b = bases;
         uint32_t* tb = bases + nb + 1;
-        uint32_t* k1f = tb + nb + 1;
-        uint32_t* tf = k1f + (uint64_t)nb * fs.k1_items;
-        uint32_t* dbt = tf + (uint64_t)nb * fs.tiles;
+        uint32_t* dbb = tb + nb + 1;
         for (uint32_t fi = 0; fi < nb; ++fi) {
             Frame& F = ftab[fi];
             std::memset(&F, 0, sizeof(F));
@@ -1923,20 +1922,15 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
             F.frame_index = (int64_t)(b0 + fi);
             k1b[fi] = F.k1_base;
             tb[fi] = F.tile_base;
-            for (uint32_t i = 0; i < fs.k1_items; ++i) k1f[F.k1_base + i] = fi;
-            for (uint32_t i = 0; i < fs.tiles; ++i) tf[F.tile_base + i] = fi;
-            uint32_t* db = dbt + (uint64_t)fi * dblocks_pf;
-            for (uint32_t ty = 0; ty < fs.ty; ty += 2)
-                for (uint32_t tx = 0; tx < fs.tx; tx += 2) *db++ = F.tile_base + ty * fs.tx + tx;
+            dbb[fi] = (uint32_t)(fi * dblocks_pf);
         }
         k1b[nb] = nb * fs.k1_items;
         tb[nb] = nb * fs.tiles;
-        const uint64_t nbases =
-            2 * ((uint64_t)nb + 1) + (uint64_t)nb * (fs.k1_items + fs.tiles + dblocks_pf);
+        dbb[nb] = (uint32_t)(nb * dblocks_pf);
         Frame* d_ftab = (Frame*)(base + L.ftab);
         uint32_t* d_bases = (uint32_t*)(base + L.bases);
         HIP_TRY(hipMemcpyAsync(d_ftab, ftab, nb * sizeof(Frame), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d_bases, bases, nbases * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d_bases, bases, 3 * ((size_t)nb + 1) * 4, hipMemcpyHostToDevice, st));
         HIP_TRY(hipEventRecord(ctx->ev_na, st));
 
         Batch b;
@@ -1944,7 +1938,7 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         b.frames = d_ftab;
         b.k1_bases = d_bases;
         b.tile_bases = d_bases + nb + 1;
-        b.k1_frame = b.tile_bases + nb + 1;
+        b.k1_frame = d_bases + 3 * ((size_t)nb + 1);
         b.tile_frame = b.k1_frame + (uint64_t)nb * fs.k1_items;
         b.dblock_tile = b.tile_frame + (uint64_t)nb * fs.tiles;
         b.dblocks = (uint32_t)(nb * dblocks_pf);
@@ -1981,6 +1975,8 @@ static gt_status locate_noalloc(gt_context* ctx, const void* frames, int in_type
         b.rows_cand = d_cand;
         b.rows_cap = rows_cap;
 
+        HIP_TRY(launch_fill_tables(b, d_bases + 2 * ((size_t)nb + 1),
+                                   (uint32_t)(fs.k1_items + fs.tiles + dblocks_pf), st));
         HIP_TRY(launch_zero_batch(b, st));
         if (generic)
             HIP_TRY(launch_bandpass_generic(b, d_geom, h * w, g.has_pct, st));
