@@ -12,9 +12,11 @@ namespace gt {
 
 // Output tile of the detect stage (K2).  It is also the grouping unit of
 // candidates: each tile owns TILE_H*TILE_W candidate slots, refinement,
-// deduplication and row emission all work tile by tile.
+// deduplication and row emission all work tile by tile.  48 columns: with
+// the dilation window's halo (up to 16 columns) a tile's staged columns are
+// the 64 lanes of a wave.
 constexpr int TILE_H = 64;
-constexpr int TILE_W = 64;
+constexpr int TILE_W = 48;
 constexpr int TILE_PX = TILE_H * TILE_W;
 
 // Bandpass (K1) column strip: 64 lanes x 4 columns.

@@ -665,7 +665,7 @@ hipError_t launch_select1(const Batch& b, const Geom& g, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // K2 detect: local maxima of the dilation window (ties kept), candidate
-// slots in raster order within each 64x64 tile.
+// slots in raster order within each TILE_W x TILE_H tile.
 // ---------------------------------------------------------------------------
 struct DetParams {
     int a;              // window start offset
@@ -799,36 +799,37 @@ __device__ __forceinline__ void pct_bin_range(int32_t k, uint32_t& base, uint32_
     width = (k >= HIST_BINS - 1) ? 0x80000000u - base : (1u << 19);
 }
 
-// One workgroup per 64x64 tile.  The tile and its window halo are staged in
-// LDS (all loads issued before the first store); the row maxima overwrite the
-// staged tile in place, so one tile needs only (64+KD-1) x (64+KD+3) floats of
-// LDS and about six workgroups share a CU, overlapping each other's load
-// latency.
+// One workgroup per 48x64 tile (TILE_W x TILE_H).  Lane c of every wave owns
+// staged column c, frame column x0 - a + c: the 64 lanes hold the tile's 48
+// columns and the dilation window's halo (KD - 1 <= 16 columns), so the
+// column maxima come out in the lanes that own the columns, and the row
+// window over them is formed by lane shifts (ds_bpermute).  The staged tile
+// ((64 + KD - 1) x 64 floats: 19.5 KB for KD = 15) is never overwritten, so
+// the walk reads it as staged; wave w tests output rows 16w .. 16w + 15.
 //
 // RI > 0: the centroid walk of every candidate (refine) runs here too, on
-// the staged tile: after the candidate test the tile's columns are staged
-// again from P (the row maxima overwrote them) and each candidate walks in
-// one lane, reading its disk from LDS (from P when the walk leaves the
-// staged area).  Every candidate walks; the percentile test on its start
-// value is applied in dedup, which reads that value from the REC_CENTER
-// plane.
+// the staged tile: each candidate walks in one lane, reading its disk from
+// LDS (from P when the walk leaves the staged area).  Every candidate walks;
+// the percentile test on its start value is applied in dedup, which reads
+// that value from the REC_CENTER plane.
 template <int KD, int RI>
 __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     constexpr int N = 16;
+    constexpr int A = (KD - 1) / 2;   // window start trunc(KD / 2 - 0.5), as dp.a
     constexpr int LR = TILE_H + KD - 1;
-    constexpr int LC = TILE_W + KD - 1;
-    constexpr int LS = ((LC + 3) / 4) * 4 + 4;
+    constexpr int LS = 64;            // staged columns: one per lane
     constexpr int NIN = N + KD - 1;
-    constexpr int NV4 = (NIN + 3) / 4;
-    constexpr int NRUN = (LR * 4 + 255) / 256;
-    static_assert(LC - 64 <= 64, "halo wider than one extra column pass");
+    // largest power of two <= KD: the span the doubling steps reach
+    constexpr int P2 = KD >= 16 ? 16 : KD >= 8 ? 8 : KD >= 4 ? 4 : KD >= 2 ? 2 : 1;
+    static_assert(TILE_W + KD - 1 <= LS, "tile and window halo must fit the 64 lanes");
     __shared__ __attribute__((aligned(16))) float L[LR * LS];
     // candidates per tile row, then (in place) the row's first slot
     __shared__ uint32_t s_rowcnt[TILE_H];
     __shared__ uint32_t s_lcnt, s_total;
 
     const int tid = threadIdx.x;
-    const int a = dp.a;
+    const int c = tid & 63;
+    const int kw = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t f = b.tile_frame[item];
     const Frame fr = b.frames[f];
@@ -836,137 +837,95 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     const int H = (int)fr.H, W = (int)fr.W;
     const int x0 = (int)(local % fr.tx) * TILE_W;
     const int y0 = (int)(local / fr.tx) * TILE_H;
+    const int v = x0 - A + c;         // this lane's frame column
     const GLOBAL float* P = (const GLOBAL float*)fr.P;
     if (tid == 0) s_lcnt = 0;
 
-    // Staging: wave w fills rows w, w+4, ...: lane = column (plus a second
-    // pass over the KD-1 halo columns).  Indices are clamped into the frame
-    // so every load is in bounds; out-of-frame cells become -inf.  All loads
-    // are issued before any store.
-    // Tiles whose staged area lies inside the frame (most of them) skip the
-    // -inf selects.  Byte offsets are 32-bit (frames hold < 2^30 pixels), so
-    // the loads take the frame base as a scalar address.
-    const bool inner = (x0 - a >= 0) && (x0 - a + LC <= W) && (y0 - a >= 0) && (y0 - a + LR <= H);
-    const GLOBAL char* Pb = (const GLOBAL char*)P;
-    auto stage_tile = [&](bool halo) {
-        constexpr int RPW = (LR + 3) / 4;
-        const int lane = tid & 63;
-        const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-        const int v1 = x0 - a + lane, v2 = v1 + 64;
-        const bool h2 = halo && lane < LC - 64;
-        if (inner) {  // workgroup-uniform
-            // every staged row lies inside the frame: one byte offset per
-            // lane, advanced by four rows per load, and LDS stores at
-            // constant offsets from one address (rows wv + 4q, q < RPW - 1,
-            // always exist; the last group only for the first waves)
-            constexpr int QF = RPW - 1;
-            const bool last = wv + 4 * QF < LR;  // wave-uniform
-            const uint32_t step = 16u * (uint32_t)W;
-            const uint32_t o = 4u * (uint32_t)((y0 - a + wv) * W + v1);
-            float f1[RPW], f2[RPW];
-#pragma unroll
-            for (int q = 0; q < QF; ++q) f1[q] = *(const GLOBAL float*)(Pb + (o + (uint32_t)q * step));
-            if (last) f1[QF] = *(const GLOBAL float*)(Pb + (o + (uint32_t)QF * step));
-            if (h2) {
-#pragma unroll
-                for (int q = 0; q < QF; ++q)
-                    f2[q] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)q * step));
-                if (last) f2[QF] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)QF * step));
-            }
-            float* d = L + wv * LS + lane;
-#pragma unroll
-            for (int q = 0; q < QF; ++q) d[4 * q * LS] = f1[q];
-            if (last) d[4 * QF * LS] = f1[QF];
-            if (h2) {
-#pragma unroll
-                for (int q = 0; q < QF; ++q) d[4 * q * LS + 64] = f2[q];
-                if (last) d[4 * QF * LS + 64] = f2[QF];
-            }
-            return;
-        }
-        const uint32_t o1 = 4u * (uint32_t)min(max(v1, 0), W - 1);
-        const uint32_t o2 = 4u * (uint32_t)min(max(v2, 0), W - 1);
-        float f1[RPW], f2[RPW];
-#pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-            const int r = min(wv + 4 * q, LR - 1);  // repeated last row: same values
-            const int u = y0 - a + r;
-            const uint32_t rb = 4u * (uint32_t)(min(max(u, 0), H - 1) * W);
-            f1[q] = *(const GLOBAL float*)(Pb + (rb + o1));
-            if (h2) f2[q] = *(const GLOBAL float*)(Pb + (rb + o2));
-        }
-        const bool c1 = v1 >= 0 && v1 < W;
-        const bool c2 = v2 >= 0 && v2 < W;
-#pragma unroll
-        for (int q = 0; q < RPW; ++q) {
-            const int r = min(wv + 4 * q, LR - 1);
-            const int u = y0 - a + r;
-            const bool ur = u >= 0 && u < H;  // wave-uniform
-            L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
-            if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
-        }
-    };
-    stage_tile(true);
-    __syncthreads();
-
-    // the max filter works on the bit patterns (see sliding_max)
-    const int32_t* Li = (const int32_t*)L;
-    // centre values of this thread's 16 outputs, read before the overwrite
-    const int c = tid & 63;
-    const int kw = __builtin_amdgcn_readfirstlane(tid >> 6);
-    int32_t center[N];
-#pragma unroll
-    for (int m = 0; m < N; ++m) center[m] = Li[(kw * N + m + a) * LS + (c + a)];
-
-    // row pass: RM[r][c] = max(L[r][c .. c+KD-1]), written over L[r][0..63]
-    int32_t y[NRUN][N];
-#pragma unroll
-    for (int k2 = 0; k2 < NRUN; ++k2) {
-        const int run = tid + k2 * 256;
-        if (run < LR * 4) {
-            const int r = run >> 2, c0 = (run & 3) * N;
-            int32_t x[NIN];
-            const int32_t* src = Li + r * LS + c0;
-#pragma unroll
-            for (int m = 0; m < NV4; ++m) {
-                const int4 q = *(const int4*)(src + 4 * m);
-                if (4 * m + 0 < NIN) x[4 * m + 0] = q.x;
-                if (4 * m + 1 < NIN) x[4 * m + 1] = q.y;
-                if (4 * m + 2 < NIN) x[4 * m + 2] = q.z;
-                if (4 * m + 3 < NIN) x[4 * m + 3] = q.w;
-            }
-            sliding_max<N, KD>(x, y[k2]);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k2 = 0; k2 < NRUN; ++k2) {
-        const int run = tid + k2 * 256;
-        if (run < LR * 4) {
-            const int r = run >> 2, c0 = (run & 3) * N;
-            int32_t* dst = (int32_t*)L + r * LS + c0;
-#pragma unroll
-            for (int m = 0; m < N / 4; ++m)
-                *(int4*)(dst + 4 * m) =
-                    make_int4(y[k2][4 * m], y[k2][4 * m + 1], y[k2][4 * m + 2], y[k2][4 * m + 3]);
-        }
-    }
-    __syncthreads();
-
-    // column pass + candidate test: lane = column, wave = 16-row run
-    const int v = x0 + c;
-    int32_t cm[N];
+    // Staging: wave kw fills rows kw, kw + 4, ...; every row is one 256-byte
+    // load across the wave.  Tiles whose staged area lies inside the frame
+    // (most of them) load with one 32-bit byte offset per lane, advanced by
+    // four rows per load (the frame base stays scalar); the others clamp
+    // their indices into the frame (every load in bounds) and store -inf
+    // outside it.  All loads are issued before any store.
     {
-        int32_t x[NIN];
+        constexpr int RPW = (LR + 3) / 4;
+        constexpr int QF = RPW - 1;
+        const GLOBAL char* Pb = (const GLOBAL char*)P;
+        const bool inner = (x0 - A >= 0) && (x0 - A + LS <= W) && (y0 - A >= 0) && (y0 - A + LR <= H);
+        float fv[RPW];
+        if (inner) {  // workgroup-uniform
+            // rows kw + 4q, q < QF, always exist; the last group only for the
+            // first waves
+            const bool last = kw + 4 * QF < LR;  // wave-uniform
+            const uint32_t step = 16u * (uint32_t)W;
+            const uint32_t o = 4u * (uint32_t)((y0 - A + kw) * W + v);
 #pragma unroll
-        for (int m = 0; m < NIN; ++m) x[m] = Li[(kw * N + m) * LS + c];
-        sliding_max<N, KD>(x, cm);
+            for (int q = 0; q < QF; ++q) fv[q] = *(const GLOBAL float*)(Pb + (o + (uint32_t)q * step));
+            if (last) fv[QF] = *(const GLOBAL float*)(Pb + (o + (uint32_t)QF * step));
+            float* d = L + kw * LS + c;
+#pragma unroll
+            for (int q = 0; q < QF; ++q) d[4 * q * LS] = fv[q];
+            if (last) d[4 * QF * LS] = fv[QF];
+        } else {
+            const uint32_t oc = 4u * (uint32_t)min(max(v, 0), W - 1);
+            const bool cin = v >= 0 && v < W;
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                const int r = min(kw + 4 * q, LR - 1);  // repeated last row: same values
+                const int u = y0 - A + r;
+                const uint32_t rb = 4u * (uint32_t)(min(max(u, 0), H - 1) * W);
+                fv[q] = *(const GLOBAL float*)(Pb + (rb + oc));
+            }
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                const int r = min(kw + 4 * q, LR - 1);
+                const int u = y0 - A + r;
+                const bool ur = u >= 0 && u < H;  // wave-uniform
+                L[r * LS + c] = (ur && cin) ? fv[q] : __uint_as_float(NEG_INF_BITS);
+            }
+        }
+    }
+    __syncthreads();
+
+    // Column pass: this lane's column over the wave's 16 output rows and the
+    // window's rows below them; the max filter works on the bit patterns
+    // (see sliding_max).  x[m + A] is the centre of output row kw * 16 + m.
+    const int32_t* Li = (const int32_t*)L;
+    int32_t x[NIN];
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) x[m] = Li[(kw * N + m) * LS + c];
+    int32_t wm[N];
+    sliding_max<N, KD>(x, wm);
+    // Row window by lane shifts: after the doubling steps wm[m] in lane j is
+    // the maximum over lanes j .. j + P2 - 1; the window of the output in
+    // lane c is lanes c - A .. c - A + KD - 1, the union of the spans that
+    // start at c - A and at c - A + KD - P2.  Lanes past 63 wrap around and
+    // feed only lanes that hold no output (c <= A + TILE_W - 1 reads at
+    // most lane A + TILE_W - 1 - A + KD - 1 <= 63).
+#pragma unroll
+    for (int s2 = 1; s2 < P2; s2 *= 2) {
+        const int src = 4 * ((c + s2) & 63);
+#pragma unroll
+        for (int m = 0; m < N; ++m) wm[m] = max(wm[m], __builtin_amdgcn_ds_bpermute(src, wm[m]));
+    }
+    {
+        constexpr int D0 = -A, D1 = KD - P2 - A;
+        const int src0 = 4 * ((c + D0) & 63), src1 = 4 * ((c + D1) & 63);
+#pragma unroll
+        for (int m = 0; m < N; ++m) {
+            const int32_t w0 = D0 == 0 ? wm[m] : __builtin_amdgcn_ds_bpermute(src0, wm[m]);
+            const int32_t w1 = D1 == D0 ? w0
+                               : D1 == 0 ? wm[m]
+                                         : __builtin_amdgcn_ds_bpermute(src1, wm[m]);
+            wm[m] = max(w0, w1);
+        }
     }
 
     // candidates: the ballots of the wave's 16 rows stay in scalar registers
-    // (bitwise tests: no short-circuit branches)
-    // (lane masks combined in scalar registers: one vector compare each)
-    const uint64_t colmask = __builtin_amdgcn_ballot_w64((v < W) & (v - dp.band >= 0) &
+    // (bitwise tests: no short-circuit branches); output lanes are
+    // A .. A + TILE_W - 1
+    const bool olane = (c >= A) & (c < A + TILE_W);
+    const uint64_t colmask = __builtin_amdgcn_ballot_w64(olane & (v < W) & (v - dp.band >= 0) &
                                                          (v + dp.band < W));
     const bool diw = dp.diw != 0;
     // rows m of the wave's run that can hold a candidate: band <= u0 + m <= H - 1 - band
@@ -977,8 +936,8 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     uint32_t mycnt = 0;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        const uint64_t ge = __builtin_amdgcn_ballot_w64(center[m] >= cm[m]);
-        const uint64_t nz = diw ? __builtin_amdgcn_ballot_w64(center[m] != 0) : ~0ull;
+        const uint64_t ge = __builtin_amdgcn_ballot_w64(x[m + A] >= wm[m]);
+        const uint64_t nz = diw ? __builtin_amdgcn_ballot_w64(x[m + A] != 0) : ~0ull;
         bal[m] = (rowbits >> m) & 1u ? (colmask & ge & nz) : 0ull;
         mycnt = (c == m) ? (uint32_t)__popcll(bal[m]) : mycnt;
     }
@@ -986,34 +945,35 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
 
     if (dp.has_pct) {
         // values in the two histogram bins holding the order statistics go to
-        // the tile's list (any order); out-of-frame centres are -inf and
-        // fall outside every range
-        // The two order statistics have adjacent ranks, so no value lies
-        // strictly between their bins: one range from the start of the lower
-        // bin to the end of the upper one holds exactly the two bins.
+        // the tile's list (any order), each pixel from the one tile whose
+        // output lanes hold it; out-of-frame centres are -inf and fall
+        // outside every range.  The two order statistics have adjacent
+        // ranks, so no value lies strictly between their bins: one range
+        // from the start of the lower bin to the end of the upper one holds
+        // exactly the two bins.
         uint32_t lb, lw, hb, hw;
         pct_bin_range(b.selstate[f * 4 + 0], lb, lw);
         pct_bin_range(b.selstate[f * 4 + 2], hb, hw);
         const uint32_t base = lw ? lb : hb;
-        const uint32_t width = hw ? (hb + hw - base) : lw;
+        const uint32_t width = olane ? (hw ? (hb + hw - base) : lw) : 0u;
         uint32_t mbits = 0;
 #pragma unroll
         for (int m = N - 1; m >= 0; --m) {
-            const uint32_t bits = (uint32_t)center[m];
+            const uint32_t bits = (uint32_t)x[m + A];
             mbits = 2u * mbits + ((bits - base < width) ? 1u : 0u);
         }
         if (__ballot(mbits != 0u)) {  // wave-uniform
             const uint32_t n = (uint32_t)__popc(mbits);
             const uint32_t inc = wave_incl_scan(n, c);
             const uint32_t tot = __shfl(inc, 63, 64);
-            uint32_t base = 0;
-            if (c == 0) base = atomicAdd(&s_lcnt, tot);
-            base = __builtin_amdgcn_readfirstlane(base);
-            uint32_t j = base + inc - n;
+            uint32_t lbase = 0;
+            if (c == 0) lbase = atomicAdd(&s_lcnt, tot);
+            lbase = __builtin_amdgcn_readfirstlane(lbase);
+            uint32_t j = lbase + inc - n;
             GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
 #pragma unroll
             for (int m = 0; m < N; ++m) {
-                if (mbits & (1u << m)) list[j++] = (uint32_t)center[m];
+                if (mbits & (1u << m)) list[j++] = (uint32_t)x[m + A];
             }
         }
     }
@@ -1032,23 +992,21 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
 #pragma unroll
     for (int m = 0; m < N; ++m) {
-        if (bal[m] == 0) continue;  // wave-uniform: most 64-pixel row pieces hold no candidate
+        if (bal[m] == 0) continue;  // wave-uniform: most row pieces hold no candidate
         const int r = kw * N + m;
         if ((bal[m] >> c) & 1ull) {
             const uint32_t rank =
                 s_rowcnt[r] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[m] >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal[m], 0u));
-            slots[rank] = (uint16_t)(r * TILE_W + c);
+            slots[rank] = (uint16_t)(r * TILE_W + (c - A));
         }
     }
 
     if constexpr (RI > 0) {
         const uint32_t total = s_total;
         if (total == 0) return;  // workgroup-uniform
-        __syncthreads();         // column pass done with L; slots visible
-        stage_tile(false);       // columns 0..63 again hold P
-        __syncthreads();
-        walk_tile<RI, LR, LC, LS>(b, dp, L, slots, total, item, P, H, W, x0, y0, a, tid);
+        __syncthreads();         // slots visible
+        walk_tile<RI, LR, LS, LS>(b, dp, L, slots, total, item, P, H, W, x0, y0, A, tid);
     }
 }
 
@@ -1089,7 +1047,7 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
         RM[idx] = m;
     }
     __syncthreads();
-    const int c = tid & 63, k = tid >> 6;
+    const int c = tid & 63, k = tid >> 6;   // lanes c >= TILE_W hold no output
     const int v = x0 + c;
     int32_t bin_lo = -1, bin_hi = -1;
     if (dp.has_pct) {
@@ -1101,10 +1059,12 @@ __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
     for (int m = 0; m < 16; ++m) {
         const int r = k * 16 + m;
         const int u = y0 + r;
-        float cmx = __uint_as_float(NEG_INF_BITS);
-        for (int i = 0; i < KD; ++i) cmx = fmaxf(cmx, RM[(r + i) * TILE_W + c]);
-        const float center = L[(r + a) * LC + (c + a)];
-        const bool inb = (u < H) && (v < W);
+        const bool inb = (c < TILE_W) && (u < H) && (v < W);
+        float cmx = __uint_as_float(NEG_INF_BITS), center = __uint_as_float(NEG_INF_BITS);
+        if (c < TILE_W) {
+            for (int i = 0; i < KD; ++i) cmx = fmaxf(cmx, RM[(r + i) * TILE_W + c]);
+            center = L[(r + a) * LC + (c + a)];
+        }
         const bool interior = (u - dp.band >= 0) && (u + dp.band < H) &&
                               (v - dp.band >= 0) && (v + dp.band < W);
         const bool cand = inb && interior && (center >= cmx) && !(dp.diw && center == 0.f);
@@ -1162,7 +1122,7 @@ __global__ __launch_bounds__(256) void k_dilate_rows(Batch b, int KD, int a) {
     const int y0 = (int)(local / fr.tx) * TILE_H;
     const int c = threadIdx.x & 63, k = threadIdx.x >> 6;
     const int v = x0 + c;
-    if (v >= W) return;
+    if (c >= TILE_W || v >= W) return;
     const float* P = fr.P;
     float* RM = fr.tmp;
     const int j0 = max(0, a - v), j1 = min(KD, W + a - v);   // in-bounds taps v - a + j
@@ -1204,7 +1164,7 @@ __global__ __launch_bounds__(256) void k_detect_plane(Batch b, DetParams dp) {
     for (int m = 0; m < 16; ++m) {
         const int r = k * 16 + m;
         const int u = y0 + r;
-        const bool inb = (u < H) && (v < W);
+        const bool inb = (c < TILE_W) && (u < H) && (v < W);
         float center = 0.f;
         bool cand = false;
         if (inb) {
@@ -1267,6 +1227,7 @@ bool detect_needs_plane(const Geom& g) {
 
 template <int KD>
 static hipError_t launch_det_t(const Batch& b, const DetParams& dp, hipStream_t s) {
+    if (dp.a != (KD - 1) / 2) return hipErrorInvalidValue;   // the kernel's window start
     switch (dp.walk_ri) {
         case 0: hipLaunchKernelGGL((k_detect<KD, 0>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
         case 1: hipLaunchKernelGGL((k_detect<KD, 1>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
@@ -1284,7 +1245,6 @@ static hipError_t launch_det_t(const Batch& b, const DetParams& dp, hipStream_t 
 static bool detect_specialised(int kd) {
     switch (kd) {
         case 4: case 5: case 7: case 8: case 9: case 11: case 12: case 14: case 15: case 16:
-        case 18: case 19: case 21:
             return true;
         default:
             return false;
@@ -1319,9 +1279,6 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
         case 14: return launch_det_t<14>(b, dp, s);
         case 15: return launch_det_t<15>(b, dp, s);
         case 16: return launch_det_t<16>(b, dp, s);
-        case 18: return launch_det_t<18>(b, dp, s);
-        case 19: return launch_det_t<19>(b, dp, s);
-        case 21: return launch_det_t<21>(b, dp, s);
         default: break;
     }
     if (detect_needs_plane(g)) {
