@@ -2162,12 +2162,45 @@ hipError_t launch_dedup(const Batch& b, const Geom& g, const Geom* d_geom, hipSt
 // offsets and sums its candidates and rows
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_tile_scan(Batch b) {
+    constexpr int PT = 8;   // tiles per thread in the one-round path
     __shared__ uint32_t s_part[4];
     __shared__ uint32_t s_cand[4];
     const uint32_t f = blockIdx.x;
     const Frame fr = b.frames[f];
     const uint32_t ntiles = fr.tx * fr.ty;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (ntiles <= 256 * PT) {  // workgroup-uniform
+        // thread t takes tiles t * PT .. t * PT + PT - 1: every load of the
+        // frame in one round, then one workgroup scan of the thread sums
+        const uint32_t k0 = (uint32_t)t * PT;
+        uint32_t kc[PT], cc[PT];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            const bool in = k0 + i < ntiles;
+            kc[i] = in ? b.tile_keep[fr.tile_base + k0 + i] : 0u;
+            cc[i] = in ? b.tile_count[fr.tile_base + k0 + i] : 0u;
+        }
+        uint32_t sum = 0, cands = 0;
+#pragma unroll
+        for (int i = 0; i < PT; ++i) { sum += kc[i]; cands += cc[i]; }
+        const uint32_t inc = wave_incl_scan(sum, lane);
+        cands = wave_sum_u(cands);
+        if (lane == 63) s_part[wave] = inc;
+        if (lane == 0) s_cand[wave] = cands;
+        __syncthreads();
+        uint32_t before = inc - sum;
+        for (int w = 0; w < wave; ++w) before += s_part[w];
+#pragma unroll
+        for (int i = 0; i < PT; ++i) {
+            if (k0 + i < ntiles) b.tile_row_off[fr.tile_base + k0 + i] = before;
+            before += kc[i];
+        }
+        if (t == 0) {
+            b.frame_rows[f] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+            b.frame_cands[f] = s_cand[0] + s_cand[1] + s_cand[2] + s_cand[3];
+        }
+        return;
+    }
     uint32_t carry = 0, cands = 0;
     for (uint32_t base = 0; base < ntiles; base += 256) {
         const uint32_t k = base + t;
@@ -2198,10 +2231,12 @@ __global__ __launch_bounds__(256) void k_tile_scan(Batch b) {
 // t itself are in raster order already.  With K[t][r] = kept rows of tile t
 // in pixel rows <= r (from dedup):
 //   rowoff[t][r] = base(tr) + sum_{t' < t} K[t'][r] + sum_{t' > t} K[t'][r - 1].
-// One wave per tile row, lane = pixel row; the loads of 32 tiles go out
-// together.
+// One wave per tile row, lane = pixel row.  Tile rows of up to 48 tiles
+// (frames up to 2304 columns) load every K once, in one round, and take
+// K[t][r - 1] from the lane below; wider ones load in chunks of 32 tiles.
 __global__ __launch_bounds__(256) void k_row_scan(Batch b) {
     constexpr int CH = 32;
+    constexpr int C1 = 48;
     const uint32_t f = blockIdx.x;
     const Frame fr = b.frames[f];
     const int lane = threadIdx.x & 63;
@@ -2210,6 +2245,28 @@ __global__ __launch_bounds__(256) void k_row_scan(Batch b) {
     const uint32_t t0 = fr.tile_base + tr * fr.tx;
     const uint32_t base = b.tile_row_off[t0];
     const uint32_t* K = b.rowk + (size_t)t0 * TILE_H + lane;
+    if (fr.tx <= C1) {  // wave-uniform
+        uint32_t kle[C1], klt[C1];
+#pragma unroll
+        for (int j = 0; j < C1; ++j) kle[j] = (uint32_t)j < fr.tx ? K[(size_t)j * TILE_H] : 0u;
+        uint32_t total = 0;
+#pragma unroll
+        for (int j = 0; j < C1; ++j) {
+            const uint32_t up = __shfl_up(kle[j], 1, 64);
+            klt[j] = lane > 0 ? up : 0u;
+            total += klt[j];
+        }
+        uint32_t pre = 0, cum = 0;
+#pragma unroll
+        for (int j = 0; j < C1; ++j) {
+            if ((uint32_t)j < fr.tx) {
+                cum += klt[j];
+                b.rowoff[(size_t)(t0 + j) * TILE_H + lane] = base + pre + (total - cum);
+                pre += kle[j];
+            }
+        }
+        return;
+    }
     uint32_t total_lt = 0;
     for (uint32_t q0 = 0; q0 < fr.tx; q0 += CH) {
         uint32_t v[CH];
@@ -2296,23 +2353,30 @@ hipError_t launch_frame_scan(const Batch& b, hipStream_t s) {
 // ecc_gpu)] in frame order, raster order of the candidate pixel
 // within a frame (row positions from k_tile_scan).
 // ---------------------------------------------------------------------------
+// Two tiles per wave, one per half (lanes 0..31 and 32..63): a tile holds
+// about a dozen candidates, and a wave's time is mostly its chain of
+// dependent loads, so fewer, fuller waves finish sooner.
 __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t item = blockIdx.x * 4 + wave;
-    if (item >= b.tiles) return;
-    const uint32_t f = b.tile_frame[item];
+    const int half = lane >> 5, hl = lane & 31;
+    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+    const uint32_t item = (blockIdx.x * 4 + wave) * 2 + half;
+    if (__ballot(item < b.tiles) == 0) return;  // wave-uniform
+    const bool tile_in = item < b.tiles;
+    const uint32_t it = tile_in ? item : 0u;   // loads stay in bounds
+    const uint32_t f = b.tile_frame[it];
     const Frame fr = b.frames[f];
-    const uint32_t local = item - fr.tile_base;
+    const uint32_t local = it - fr.tile_base;
     const int W = (int)fr.W;
     const int tx = (int)(local % fr.tx), ty = (int)(local / fr.tx);
-    const uint32_t cnt = b.tile_count[item];
-    const size_t slot0 = (size_t)item * TILE_PX;
+    const uint32_t cnt = tile_in ? b.tile_count[it] : 0u;
+    const size_t slot0 = (size_t)it * TILE_PX;
     const uint64_t cap = b.slot_cap;
     const int nc = g->ncols;
     const uint64_t fbase = b.frame_row_base[f];
     uint32_t before = 0;  // kept rows of this tile before the chunk
-    for (uint32_t ib = 0; ib < cnt; ib += 64) {
-        const uint32_t i = ib + lane;
+    for (uint32_t ib = 0; ib < cnt; ib += 32) {
+        const uint32_t i = ib + (uint32_t)hl;
         // the keep flag, the slot and the position record of every candidate
         // of the chunk are loaded together (all in bounds for i < cnt); only
         // the row offset waits for the slot
@@ -2324,9 +2388,9 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
         const float ryv = b.rec[REC_Y * cap + s];
         const float rxv = b.rec[REC_X * cap + s];
         const bool k = in && kf;
-        const uint64_t msk = __ballot(k);
+        const uint64_t msk = __ballot(k) & hmask;
         if (k) {
-            const uint64_t r = fbase + b.rowoff[(size_t)item * TILE_H + l / TILE_W] + before +
+            const uint64_t r = fbase + b.rowoff[(size_t)it * TILE_H + l / TILE_W] + before +
                                (uint64_t)__popcll(msk & lanemask_lt(lane));
             if (r < b.rows_cap) {
                 float* o = b.rows + r * (uint64_t)nc;
@@ -2357,7 +2421,7 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
 
 hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
     (void)g;
-    hipLaunchKernelGGL(k_emit, dim3((b.tiles + 3) / 4), dim3(256), 0, s, b, d_geom);
+    hipLaunchKernelGGL(k_emit, dim3((b.tiles + 7) / 8), dim3(256), 0, s, b, d_geom);
     return hipGetLastError();
 }
 
