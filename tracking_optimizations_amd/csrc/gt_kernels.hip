@@ -2353,14 +2353,16 @@ hipError_t launch_frame_scan(const Batch& b, hipStream_t s) {
 // ecc_gpu)] in frame order, raster order of the candidate pixel
 // within a frame (row positions from k_tile_scan).
 // ---------------------------------------------------------------------------
-// Two tiles per wave, one per half (lanes 0..31 and 32..63): a tile holds
-// about a dozen candidates, and a wave's time is mostly its chain of
-// dependent loads, so fewer, fuller waves finish sooner.
+// EMIT_TPW tiles per wave, EMIT_LPT lanes each: a tile holds about a dozen
+// candidates, and a wave's time is mostly its chain of dependent loads, so
+// fewer, fuller waves finish sooner.
+constexpr int EMIT_TPW = 4;
+constexpr int EMIT_LPT = 64 / EMIT_TPW;
 __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ g) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int half = lane >> 5, hl = lane & 31;
-    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
-    const uint32_t item = (blockIdx.x * 4 + wave) * 2 + half;
+    const int part = lane / EMIT_LPT, hl = lane % EMIT_LPT;
+    const uint64_t hmask = (~0ull >> (64 - EMIT_LPT)) << (part * EMIT_LPT);
+    const uint32_t item = (blockIdx.x * 4 + wave) * EMIT_TPW + part;
     if (__ballot(item < b.tiles) == 0) return;  // wave-uniform
     const bool tile_in = item < b.tiles;
     const uint32_t it = tile_in ? item : 0u;   // loads stay in bounds
@@ -2375,7 +2377,7 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
     const int nc = g->ncols;
     const uint64_t fbase = b.frame_row_base[f];
     uint32_t before = 0;  // kept rows of this tile before the chunk
-    for (uint32_t ib = 0; ib < cnt; ib += 32) {
+    for (uint32_t ib = 0; ib < cnt; ib += EMIT_LPT) {
         const uint32_t i = ib + (uint32_t)hl;
         // the keep flag, the slot and the position record of every candidate
         // of the chunk are loaded together (all in bounds for i < cnt); only
@@ -2421,7 +2423,8 @@ __global__ __launch_bounds__(256) void k_emit(Batch b, const Geom* __restrict__ 
 
 hipError_t launch_emit(const Batch& b, const Geom& g, const Geom* d_geom, hipStream_t s) {
     (void)g;
-    hipLaunchKernelGGL(k_emit, dim3((b.tiles + 7) / 8), dim3(256), 0, s, b, d_geom);
+    hipLaunchKernelGGL(k_emit, dim3((b.tiles + 4 * EMIT_TPW - 1) / (4 * EMIT_TPW)), dim3(256), 0, s,
+                       b, d_geom);
     return hipGetLastError();
 }
 
