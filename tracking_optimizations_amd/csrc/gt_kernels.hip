@@ -902,18 +902,25 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     // start at c - A and at c - A + KD - P2.  Lanes past 63 wrap around and
     // feed only lanes that hold no output (c <= A + TILE_W - 1 reads at
     // most lane A + TILE_W - 1 - A + KD - 1 <= 63).
-    // the first step by a DPP wave shift (lane j reads lane j + 1; lane 63
-    // reads 0, and lane 63 feeds no output), the others through ds_bpermute
-    if constexpr (P2 >= 2) {
+    // Shifts by 1 and 2 are DPP wave shifts (lane j reads lane j + 1; lane
+    // 63 reads 0, and lanes past the last output's window feed no output),
+    // folded into the max; longer shifts go through ds_bpermute (measured:
+    // a shift by 4 as four DPP moves is slower).
 #pragma unroll
-        for (int m = 0; m < N; ++m)
-            wm[m] = max(wm[m], __builtin_amdgcn_mov_dpp(wm[m], 0x130, 0xf, 0xf, true));
-    }
+    for (int s2 = 1; s2 < P2; s2 *= 2) {
+        if (s2 <= 2) {
 #pragma unroll
-    for (int s2 = 2; s2 < P2; s2 *= 2) {
-        const int src = 4 * ((c + s2) & 63);
+            for (int m = 0; m < N; ++m) {
+                int32_t t = wm[m];
 #pragma unroll
-        for (int m = 0; m < N; ++m) wm[m] = max(wm[m], __builtin_amdgcn_ds_bpermute(src, wm[m]));
+                for (int q = 0; q < s2; ++q) t = __builtin_amdgcn_mov_dpp(t, 0x130, 0xf, 0xf, true);
+                wm[m] = max(wm[m], t);
+            }
+        } else {
+            const int src = 4 * ((c + s2) & 63);
+#pragma unroll
+            for (int m = 0; m < N; ++m) wm[m] = max(wm[m], __builtin_amdgcn_ds_bpermute(src, wm[m]));
+        }
     }
     {
         constexpr int D0 = -A, D1 = KD - P2 - A;
