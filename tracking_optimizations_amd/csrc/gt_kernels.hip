@@ -391,10 +391,11 @@ void k_bandpass(Batch b, K1Params kp) {
                         if (coob || !row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
                         *(f4v*)(bg_ + 4 * t) = m;
                     } else {
-                        // every chunk of an interior strip is its own
+                        // every chunk of an interior strip is its own, so the
+                        // two copies agree on rows inside the frame and the
+                        // Gaussian copy of a row outside it is zero: one copy
+                        // is staged
                         *(f4v*)(bb_ + 4 * t) = m;
-                        if (!row_in) m = f4v{0.f, 0.f, 0.f, 0.f};
-                        *(f4v*)(bg_ + 4 * t) = m;
                     }
                 }
                 if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(inb, row_off(vr + PF) + cboff);
@@ -415,21 +416,32 @@ void k_bandpass(Batch b, K1Params kp) {
                         tb[q] = bx;
                     }
                 }
-                // Gaussian row pass: zeros outside the frame (skipped taps); the
-                // second copy is read in every strip so no registers are merged
-                float u[20];
-                lds_ld20(bg_ + rbase, u);
+                // Gaussian row pass: zeros outside the frame (skipped taps).
+                // A row outside the frame sums to +0 (FMAs of +0 into +0)
+                // and so divides to +0: interior strips skip it.
+                auto gauss = [&](const float (&u)[20], float (&tg)[4]) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float gs = 0.f;
+#pragma unroll
+                        for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
+                        tg[q] = cdiv_t(gs, kp.gn, kp.rgn, FAST);
+                    }
+                };
                 float tg[4];
+                if constexpr (BORDER) {
+                    float u[20];
+                    lds_ld20(bg_ + rbase, u);
+                    gauss(u, tg);
+                } else if (row_in) {  // wave-uniform
+                    gauss(v, tg);
+                } else {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    float gs = 0.f;
-#pragma unroll
-                    for (int i = -R; i <= R; ++i) gs = __builtin_fmaf(w[i + R], u[8 + q + i], gs);
-                    tg[q] = gs;
+                    for (int q = 0; q < 4; ++q) tg[q] = 0.f;
                 }
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const float gq = cdiv_t(tg[q], kp.gn, kp.rgn, FAST);
+                    const float gq = tg[q];
                     vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
                     rg[s][q] = gq;
                     rb[s][q] = tb[q];
