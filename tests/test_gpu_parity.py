@@ -336,26 +336,34 @@ def test_host_result_blocks_reused(ctx):
     assert np.array_equal(rows, rows2) and np.array_equal(cand, cand2)
 
 
-def _max_staged_records(frame, kw, tile=64):
-    """Largest number of pre-dedup records any tile's dedup has to stage: the
-    valid records of the tiles within its window (separation + 2 (max_iter +
-    rint) + 1 px) that lie within separation + 1 of the bounding box of the
-    tile's own records.  Computed on the CPU from the oracle's candidates."""
+TILE_H, TILE_W = 64, 48   # detect tile (rows, columns); dedup works on 2 x 2 blocks of them
+
+
+def _max_staged_records(frame, kw):
+    """Largest number of pre-dedup records any dedup block (2 x 2 tiles of
+    64 rows x 48 columns) has to stage: the valid records of the tiles within
+    its window (separation + 2 (max_iter + rint) + 1 px, rounded out to
+    whole tiles past the block) that lie within separation + 1 of the
+    bounding box of the block's own records; and the largest number of
+    window tiles.  Computed on the CPU from the oracle's candidates."""
     p = _oracle_params(kw)
     cand, y, x, m = oracle.candidates_frame(frame.astype(np.float32), p)
     H, W = frame.shape
     g = oracle.geometry(p)
     sep = p.c().separation
     D = np.ceil(sep) + 2.0 * (p.max_iterations + g.rint) + 1.0
-    rt = int(D // tile) + 1
-    ty, tx = (cand // W) // tile, (cand % W) // tile
+    rty, rtx = int(D // TILE_H) + 1, int(D // TILE_W) + 1
+    nty, ntx = (H + TILE_H - 1) // TILE_H, (W + TILE_W - 1) // TILE_W
+    ty, tx = (cand // W) // TILE_H, (cand % W) // TILE_W
+    by, bx = ty // 2, tx // 2
     best, ntl_max = 0, 0
-    for a in np.unique(ty):
-        for b in np.unique(tx[ty == a]):
-            own = (ty == a) & (tx == b)
-            win = (np.abs(ty - a) <= rt) & (np.abs(tx - b) <= rt)
-            ntl_max = max(ntl_max, (min(a + rt, (H - 1) // tile) - max(a - rt, 0) + 1) *
-                          (min(b + rt, (W - 1) // tile) - max(b - rt, 0) + 1))
+    for a in np.unique(by):
+        for b in np.unique(bx[by == a]):
+            own = (by == a) & (bx == b)
+            y_lo, y_hi = max(0, 2 * a - rty), min(nty - 1, 2 * a + 1 + rty)
+            x_lo, x_hi = max(0, 2 * b - rtx), min(ntx - 1, 2 * b + 1 + rtx)
+            win = (ty >= y_lo) & (ty <= y_hi) & (tx >= x_lo) & (tx <= x_hi)
+            ntl_max = max(ntl_max, (y_hi - y_lo + 1) * (x_hi - x_lo + 1))
             mg = sep + 1.0
             box = ((y >= y[own].min() - mg) & (y <= y[own].max() + mg) &
                    (x >= x[own].min() - mg) & (x <= x[own].max() + mg))
@@ -369,8 +377,8 @@ def test_dense_dedup_stages_in_rounds(ctx):
     rows still equal the oracle."""
     rng = np.random.default_rng(21)
     frame = rng.poisson(300, (1, 384, 320)).astype(np.uint16)
-    for kw in (dict(diameter=3, separation=3, smoothing_size=3),      # two rounds
-               dict(diameter=3, separation=2, smoothing_size=5)):     # seven rounds
+    for kw in (dict(diameter=3, separation=3, smoothing_size=3),      # ~970 records: 4 rounds
+               dict(diameter=3, separation=2, smoothing_size=5)):     # ~4600: 19 rounds
         staged, _ = _max_staged_records(frame[0], kw)
         print(f"[parity] dense dedup {kw}: up to {staged} staged records per tile", flush=True)
         assert staged > 256, staged
@@ -384,9 +392,9 @@ def test_dense_dedup_stages_in_rounds(ctx):
                                 dict(diameter=9, separation=200, percentile=64)],
                          ids=["max_iterations_100", "separation_200"])
 def test_wide_dedup_window(ctx, kw):
-    """Dedup windows wider than 64 tiles (separation + 2 (max_iterations +
-    rint) + 1 > 192 px) on 1024x1024 frames: the neighbour tiles are taken in
-    groups of 64, and rows equal the oracle."""
+    """Dedup windows of more than 64 tiles (separation + 2 (max_iterations +
+    rint) + 1 of about 200 px and more) on 1024x1024 frames: the neighbour
+    tiles are taken in groups of 64, and rows equal the oracle."""
     spec = synth.link_1024(seed=5, n_frames=2)
     frames = synth.generate_host(ctx, spec)
     _, ntl = _max_staged_records(frames[0], kw)
