@@ -46,6 +46,14 @@ __device__ __forceinline__ uint32_t ykey(float y) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// step_of[i] = the step row i belongs to: one workgroup per step fills its
+// rows (steps may be empty).
+__global__ void k_link_stepof(const uint32_t* __restrict__ step_off, uint32_t* __restrict__ step_of) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t a = step_off[s], e = step_off[s + 1];
+    for (uint32_t i = a + threadIdx.x; i < e; i += blockDim.x) step_of[i] = s;
+}
+
 __global__ void k_link_keys(const float* __restrict__ ys, uint32_t* __restrict__ keys,
                             uint32_t* __restrict__ vals, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,11 +203,15 @@ struct Buf {
 
 }  // namespace
 
-// Device buffers and pinned host tables of the linker, kept by the context
-// between calls (grown, never shrunk).
+// Device buffers, pinned host tables and host scratch of the linker, kept by
+// the context between calls (grown, never shrunk), so repeated calls touch
+// no fresh pages.
 struct LinkCache {
     Buf<false> y, x, stepof, stepoff, k0, v0, k1, v1, cnt, off, cd, dist, sort_tmp, scan_tmp;
     Buf<true> off_h, cd_h, dist_h;
+    std::vector<float> ry, rx;      // link_rows: the y and x columns
+    std::vector<float> cy, cx;      // link_chunk: carried entries, then the chunk
+    std::vector<uint32_t> stamp;    // link_steps: the per-id stamps of a call
     hipStream_t stream = nullptr;   // for linking beside the locate streams
     ~LinkCache() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -245,24 +257,40 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
     // the chunk's steps; a carried particle's step keeps its age relative to
     // the chunk, so (row, age) candidate lists mean the same as in one call.
     std::vector<uint32_t> step_off(1, 0);
-    std::vector<float> ysv, xsv;
     std::vector<uint64_t> carry_id;
-    auto carry = [&](const std::vector<LinkState::Carry>& v) {
-        for (const auto& e : v) {
-            ysv.push_back(e.y);
-            xsv.push_back(e.x);
-            carry_id.push_back(e.id);
+    uint32_t ncar = 0;
+    for (uint32_t k = 0; k < memory; ++k) ncar += (uint32_t)state->ring[k].size();
+    ncar += (uint32_t)state->prev.size();
+    // with nothing carried in (a first chunk) the chunk's own columns serve
+    const float* ys = ys_c;
+    const float* xs = xs_c;
+    {
+        std::vector<float>& ysv = cache->cy;
+        std::vector<float>& xsv = cache->cx;
+        if (ncar) {
+            ysv.resize((size_t)ncar + nc);
+            xsv.resize((size_t)ncar + nc);
         }
-        step_off.push_back((uint32_t)ysv.size());
-    };
-    for (uint32_t k = 0; k < memory; ++k) carry(state->ring[k]);
-    carry(state->prev);
-    const uint32_t ncar = (uint32_t)ysv.size();
-    ysv.insert(ysv.end(), ys_c, ys_c + nc);
-    xsv.insert(xsv.end(), xs_c, xs_c + nc);
+        uint32_t w = 0;
+        auto carry = [&](const std::vector<LinkState::Carry>& v) {
+            for (const auto& e : v) {
+                ysv[w] = e.y;
+                xsv[w] = e.x;
+                carry_id.push_back(e.id);
+                ++w;
+            }
+            step_off.push_back(w);
+        };
+        for (uint32_t k = 0; k < memory; ++k) carry(state->ring[k]);
+        carry(state->prev);
+        if (ncar) {
+            std::copy(ys_c, ys_c + nc, ysv.begin() + ncar);
+            std::copy(xs_c, xs_c + nc, xsv.begin() + ncar);
+            ys = ysv.data();
+            xs = xsv.data();
+        }
+    }
     for (uint32_t k = 1; k <= nsc; ++k) step_off.push_back(ncar + soff_c[k]);
-    const float* ys = ysv.data();
-    const float* xs = xsv.data();
     const uint32_t nsteps = (uint32_t)step_off.size() - 1;
     const uint32_t first = memory + 1;   // first step of the chunk
     const uint32_t n = step_off.back();
@@ -284,9 +312,6 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
     float* cdist_h = nullptr;
     if (n == 0 || nsteps <= 1) std::fill(off_h, off_h + nt + 1, 0u);   // no pairs
     if (n > 0 && nsteps > 1) {
-        std::vector<uint32_t> step_of(n);
-        for (uint32_t s = 0; s < nsteps; ++s)
-            for (uint32_t i = step_off[s]; i < step_off[s + 1]; ++i) step_of[i] = s;
         LK_TRY(C.y.ensure((size_t)n * 4));
         LK_TRY(C.x.ensure((size_t)n * 4));
         LK_TRY(C.stepof.ensure((size_t)n * 4));
@@ -304,9 +329,10 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
         uint32_t *d_cnt = C.cnt.as<uint32_t>(), *d_off = C.off.as<uint32_t>();
         LK_TRY(hipMemcpyAsync(d_y, ys, (size_t)n * 4, hipMemcpyHostToDevice, st));
         LK_TRY(hipMemcpyAsync(d_x, xs, (size_t)n * 4, hipMemcpyHostToDevice, st));
-        LK_TRY(hipMemcpyAsync(d_stepof, step_of.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
         LK_TRY(hipMemcpyAsync(d_stepoff, step_off.data(), (size_t)(nsteps + 1) * 4,
                               hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_link_stepof, dim3(nsteps), dim3(256), 0, st, d_stepoff, d_stepof);
+        LK_TRY(hipGetLastError());
         hipLaunchKernelGGL(k_link_keys, dim3((n + 255) / 256), dim3(256), 0, st, d_y, d_k0, d_v0,
                            n);
         LK_TRY(hipGetLastError());
@@ -559,7 +585,14 @@ hipError_t link_steps(hipStream_t st, LinkCache* cache, const float* ys, const f
     // never expire within the call
     const uint32_t nsteps = (uint32_t)step_off.size() - 1;
     std::unique_ptr<LinkState> state(link_state_create(std::min(memory, nsteps)));
-    return link_chunk(st, cache, state.get(), ys, xs, step_off, search_range, ids, stats, err);
+    // the stamps of a fresh state start at zero: the cache's vector, cleared
+    // (no fresh pages), lent to the state for the call
+    state->stamp.swap(cache->stamp);
+    state->stamp.assign((size_t)step_off.back() + 1, 0u);
+    const hipError_t e =
+        link_chunk(st, cache, state.get(), ys, xs, step_off, search_range, ids, stats, err);
+    state->stamp.swap(cache->stamp);
+    return e;
 }
 
 hipError_t link_rows(hipStream_t st, LinkCache* cache, const float* rows, uint64_t n_rows,
@@ -575,7 +608,10 @@ hipError_t link_rows(hipStream_t st, LinkCache* cache, const float* rows, uint64
     // against 0.0 (a first frame other than 0 yields an empty first step);
     // no rows, no steps.
     std::vector<uint32_t> step_off(1, 0);
-    std::vector<float> ys(n_rows), xs(n_rows);
+    std::vector<float>& ys = cache->ry;
+    std::vector<float>& xs = cache->rx;
+    ys.resize(n_rows);
+    xs.resize(n_rows);
     float cur = 0.0f;
     for (uint64_t i = 0; i < n_rows; ++i) {
         const float fr = rows[i * ncols + 0];
