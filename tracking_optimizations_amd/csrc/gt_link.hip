@@ -65,7 +65,9 @@ __global__ void k_link_keys(const float* __restrict__ ys, uint32_t* __restrict__
 
 // One thread per (source particle p, age j).  Without `off` it counts the
 // candidates into counts[t]; with `off` it writes them from off[t] on, as
-// (destination index within the target step, squared distance).
+// (destination index within the target step, squared distance), in
+// ascending destination order (the search finds them in y order; the thread
+// sorts its own short list).
 __global__ void k_link_pairs(const float* __restrict__ ys, const float* __restrict__ xs,
                              const uint32_t* __restrict__ step_of,
                              const uint32_t* __restrict__ step_off, uint32_t nsteps,
@@ -105,6 +107,19 @@ __global__ void k_link_pairs(const float* __restrict__ ys, const float* __restri
                     out_dist[base + c] = dist;
                 }
                 ++c;
+            }
+        }
+        if (off) {
+            for (uint32_t q = base + 1; q < base + c; ++q) {  // insertion sort by destination
+                const uint32_t dq = out_d[q];
+                const float vq = out_dist[q];
+                uint32_t k = q;
+                for (; k > base && out_d[k - 1] > dq; --k) {
+                    out_d[k] = out_d[k - 1];
+                    out_dist[k] = out_dist[k - 1];
+                }
+                out_d[k] = dq;
+                out_dist[k] = vq;
             }
         }
     }
@@ -437,23 +452,12 @@ hipError_t link_chunk(hipStream_t st, LinkCache* cache, LinkState* state, const 
         // each source's candidate list [b, e) for this step
         lists.resize(ns);
         for (uint32_t si = 0; si < ns; ++si) list_of(prev[si], t, lists[si].first, lists[si].second);
-        // every source's list in ascending destination order (each list
-        // belongs to one step only, so it is sorted in place), and the
-        // destination -> source lists in ascending source order
+        // every source's list is in ascending destination order (sorted by
+        // k_link_pairs); the destination -> source lists in ascending source
+        // order
         d2s_off.assign((size_t)nd + 1, 0);
         for (uint32_t si = 0; si < ns; ++si) {
             const uint32_t b = lists[si].first, e = lists[si].second;
-            for (uint32_t q = b + 1; q < e; ++q) {   // insertion sort, lists are short
-                const uint32_t d = cd_h[q];
-                const float v = cdist_h[q];
-                uint32_t k = q;
-                for (; k > b && cd_h[k - 1] > d; --k) {
-                    cd_h[k] = cd_h[k - 1];
-                    cdist_h[k] = cdist_h[k - 1];
-                }
-                cd_h[k] = d;
-                cdist_h[k] = v;
-            }
             for (uint32_t q = b; q < e; ++q) ++d2s_off[cd_h[q] + 1];
         }
         for (uint32_t d = 0; d < nd; ++d) d2s_off[d + 1] += d2s_off[d];
