@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "gt_link_solve.h"
+
 namespace gt {
 
 // Output tile of the detect stage (K2).  It is also the grouping unit of
@@ -200,19 +202,7 @@ void link_cache_destroy(LinkCache* c);
 // A stream of the cache's own (default priority), created on first use with
 // the calling thread's device current.
 hipError_t link_cache_stream(LinkCache* c, hipStream_t* out);
-// Linker state between chunks of steps: the last step's particles and the
-// memory entries (oldest step first), the next id, the steps seen so far.
-struct LinkState {
-    struct Carry {
-        float y, x;
-        uint64_t id;
-    };
-    uint32_t memory = 0;
-    uint64_t counter = 0, steps_done = 0;
-    std::vector<Carry> prev;
-    std::vector<std::vector<Carry>> ring;   // `memory` steps, oldest first
-    std::vector<uint32_t> stamp;            // per id, solver scratch
-};
+// Linker state between chunks of steps (LinkState, gt_link_solve.h).
 LinkState* link_state_create(uint32_t memory);
 void link_state_destroy(LinkState* s);
 // Links the steps of one chunk (soff: chunk-local step offsets) after the
