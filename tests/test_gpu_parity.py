@@ -405,6 +405,36 @@ def test_wide_dedup_window(ctx, kw):
     _check(rows, cand, ref_rows, ref_cand, f"wide window {kw}")
 
 
+def _boundary_frames(w, h=130, seed=48):
+    """One uint16 frame of width w with spots centred on and beside the
+    48-column tile boundaries and the 64-row one, plus two at the right
+    edge, on a Poisson background."""
+    rng = np.random.default_rng(seed + w)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = rng.poisson(80, (h, w)).astype(np.float64)
+    for cx in (20.0, 47.0, 48.0, 71.5, 95.5, 96.0, w - 1.0, w - 3.5):
+        for cy in (10.0, 63.5, 64.0, 100.0):
+            img += 900.0 * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2 * 1.6 ** 2))
+    return np.floor(img).astype(np.uint16)[None]
+
+
+@pytest.mark.parametrize("w", [97, 113, 143])
+def test_tile_column_boundaries(ctx, w):
+    """Detect tiles are 48 columns wide, their staged columns (tile and
+    window halo) the 64 lanes of a wave: spots on and beside the tile
+    boundaries, frames leaving 1, 17 and 47 columns in the last tile, and two
+    window sizes (KD 7 and 15) give the oracle's candidates and rows."""
+    frames = _boundary_frames(w)
+    for kw in (dict(diameter=7, separation=5),
+               dict(diameter=11, separation=11, smoothing_size=11, percentile=64)):
+        print(f"[parity] tile boundaries w={w} {kw}", flush=True)
+        ref_rows, ref_cand = _oracle_movie(frames, kw)
+        assert len(ref_cand) > 0
+        for plain in (False, True):
+            rows, cand, _, _ = locate(frames, make_params(**kw), plain=plain, ctx=ctx)
+            _check(rows, cand, ref_rows, ref_cand, f"tile boundaries w={w} {kw} plain={plain}")
+
+
 def test_uneven_workload_content(ctx):
     """Frames of the uneven benchmark batch itself (its generator: one spot per
     800 px^2, sigma U(1.2, 2.5), lognormal peaks, background U(50, 300) per
