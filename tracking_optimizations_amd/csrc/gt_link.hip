@@ -13,15 +13,11 @@
 // every (p, j) its exact candidate list, so a memory entry of any age finds
 // its candidates in the same table.
 //
-// The subnet solver then runs on the host in the order of Linker::advance /
-// link (linking.rs:234-492): memory entries appended oldest first, depth-
-// first subnets from each destination in order, one-to-one pairs taken
-// directly, exhaustive minimum-cost assignment (null cost r*r, first
-// strictly better score kept, binary32 running score), new ids in
-// destination order, unmatched sources of the previous step kept as memory.
-// A source's candidates are taken in ascending destination order; the
-// reference takes its kd-tree's traversal order, which only decides between
-// assignments of equal binary32 cost.
+// The subnet solver (gt_link_solve.h) then runs on the host in the order of
+// Linker::advance / link (linking.rs:234-492).  A source's candidates are
+// taken in ascending destination order (each thread sorts its own list);
+// the reference takes its kd-tree's traversal order, which only decides
+// between assignments of equal binary32 cost.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
