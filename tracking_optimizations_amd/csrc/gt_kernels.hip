@@ -818,8 +818,8 @@ __device__ __forceinline__ void pct_bin_range(int32_t k, uint32_t& base, uint32_
 // window over them is formed by lane shifts (DPP and ds_bpermute).  The
 // staged tile ((64 + KD - 1) x 64 floats: 19.5 KB for KD = 15) is never
 // overwritten, so the walk reads it as staged; wave w tests output rows
-// 16w .. 16w + 15.  About 20 KB of LDS and 59 VGPRs let eight workgroups
-// share a CU.
+// 16w .. 16w + 15.  With about 20 KB of LDS and 59 VGPRs, seven workgroups
+// share a CU (the row ballots' scalar registers set that).
 //
 // RI > 0: the centroid walk of every candidate (refine) runs here too, on
 // the staged tile: each candidate walks in one lane, reading its disk from
