@@ -45,6 +45,9 @@ CASES = {
     "small_sep": dict(diameter=7, separation=5, smoothing_size=9),
     "wide_smooth": dict(diameter=9, separation=9, smoothing_size=17, noise_size=2.0),
     "wide_sep": dict(diameter=9, separation=20),
+    # the default separation of diameter 13 (14): dilation window 19, the
+    # row-first detect kernel (windows wider than the column-first one's lanes)
+    "d13_default_sep": dict(diameter=13),
     "pct0": dict(diameter=9, separation=9, percentile=0),
     "pct100": dict(diameter=9, separation=9, percentile=100),
     # dilation halo (3) narrower than the disk radius (5): walks near tile
@@ -422,11 +425,13 @@ def _boundary_frames(w, h=130, seed=48):
 def test_tile_column_boundaries(ctx, w):
     """Detect tiles are 48 columns wide, their staged columns (tile and
     window halo) the 64 lanes of a wave: spots on and beside the tile
-    boundaries, frames leaving 1, 17 and 47 columns in the last tile, and two
-    window sizes (KD 7 and 15) give the oracle's candidates and rows."""
+    boundaries, frames leaving 1, 17 and 47 columns in the last tile, and
+    window sizes 7, 15 (column-first kernel) and 19 (row-first kernel) give
+    the oracle's candidates and rows."""
     frames = _boundary_frames(w)
     for kw in (dict(diameter=7, separation=5),
-               dict(diameter=11, separation=11, smoothing_size=11, percentile=64)):
+               dict(diameter=11, separation=11, smoothing_size=11, percentile=64),
+               dict(diameter=13, separation=14, percentile=64)):
         print(f"[parity] tile boundaries w={w} {kw}", flush=True)
         ref_rows, ref_cand = _oracle_movie(frames, kw)
         assert len(ref_cand) > 0

@@ -1038,6 +1038,264 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     }
 }
 
+// Detect for dilation windows too wide for the lanes of k_detect (KD > 16,
+// separation 13 and more), row first: one workgroup per 48x64 tile, the tile
+// and its window halo staged in LDS (all loads issued before the first
+// store); the row maxima overwrite the staged tile in place, so one tile
+// needs (64+KD-1) x (48+KD+3) floats of LDS and six workgroups share a CU.
+// Lane c holds output column c (lanes 48..63 hold none in the column pass).
+//
+// RI > 0: the centroid walk of every candidate (refine) runs here too, on
+// the staged tile: after the candidate test the tile's columns are staged
+// again from P (the row maxima overwrote them) and each candidate walks in
+// one lane, reading its disk from LDS (from P when the walk leaves the
+// staged area).  Every candidate walks; the percentile test on its start
+// value is applied in dedup, which reads that value from the REC_CENTER
+// plane.
+template <int KD, int RI>
+__global__ __launch_bounds__(256) void k_detect_rows(Batch b, DetParams dp) {
+    constexpr int N = 16;
+    constexpr int LR = TILE_H + KD - 1;
+    constexpr int LC = TILE_W + KD - 1;
+    constexpr int LS = ((LC + 3) / 4) * 4 + 4;
+    constexpr int NIN = N + KD - 1;
+    constexpr int NV4 = (NIN + 3) / 4;
+    constexpr int RUNS = TILE_W / N;              // 16-column runs of an output row
+    constexpr int NRUN = (LR * RUNS + 255) / 256;
+    static_assert(LC - 64 <= 64, "halo wider than one extra column pass");
+    static_assert(LC >= 64, "the first staging pass fills 64 columns");
+    __shared__ __attribute__((aligned(16))) float L[LR * LS];
+    // candidates per tile row, then (in place) the row's first slot
+    __shared__ uint32_t s_rowcnt[TILE_H];
+    __shared__ uint32_t s_lcnt, s_total;
+
+    const int tid = threadIdx.x;
+    const int a = dp.a;
+    const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t f = b.tile_frame[item];
+    const Frame fr = b.frames[f];
+    const uint32_t local = item - fr.tile_base;
+    const int H = (int)fr.H, W = (int)fr.W;
+    const int x0 = (int)(local % fr.tx) * TILE_W;
+    const int y0 = (int)(local / fr.tx) * TILE_H;
+    const GLOBAL float* P = (const GLOBAL float*)fr.P;
+    if (tid == 0) s_lcnt = 0;
+
+    // Staging: wave w fills rows w, w+4, ...: lane = column (plus a second
+    // pass over the KD-1 halo columns).  Indices are clamped into the frame
+    // so every load is in bounds; out-of-frame cells become -inf.  All loads
+    // are issued before any store.
+    // Tiles whose staged area lies inside the frame (most of them) skip the
+    // -inf selects.  Byte offsets are 32-bit (frames hold < 2^30 pixels), so
+    // the loads take the frame base as a scalar address.
+    const bool inner = (x0 - a >= 0) && (x0 - a + LC <= W) && (y0 - a >= 0) && (y0 - a + LR <= H);
+    const GLOBAL char* Pb = (const GLOBAL char*)P;
+    auto stage_tile = [&](bool halo) {
+        constexpr int RPW = (LR + 3) / 4;
+        const int lane = tid & 63;
+        const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int v1 = x0 - a + lane, v2 = v1 + 64;
+        const bool h2 = halo && lane < LC - 64;
+        if (inner) {  // workgroup-uniform
+            // every staged row lies inside the frame: one byte offset per
+            // lane, advanced by four rows per load, and LDS stores at
+            // constant offsets from one address (rows wv + 4q, q < RPW - 1,
+            // always exist; the last group only for the first waves)
+            constexpr int QF = RPW - 1;
+            const bool last = wv + 4 * QF < LR;  // wave-uniform
+            const uint32_t step = 16u * (uint32_t)W;
+            const uint32_t o = 4u * (uint32_t)((y0 - a + wv) * W + v1);
+            float f1[RPW], f2[RPW];
+#pragma unroll
+            for (int q = 0; q < QF; ++q) f1[q] = *(const GLOBAL float*)(Pb + (o + (uint32_t)q * step));
+            if (last) f1[QF] = *(const GLOBAL float*)(Pb + (o + (uint32_t)QF * step));
+            if (h2) {
+#pragma unroll
+                for (int q = 0; q < QF; ++q)
+                    f2[q] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)q * step));
+                if (last) f2[QF] = *(const GLOBAL float*)(Pb + (o + 256u + (uint32_t)QF * step));
+            }
+            float* d = L + wv * LS + lane;
+#pragma unroll
+            for (int q = 0; q < QF; ++q) d[4 * q * LS] = f1[q];
+            if (last) d[4 * QF * LS] = f1[QF];
+            if (h2) {
+#pragma unroll
+                for (int q = 0; q < QF; ++q) d[4 * q * LS + 64] = f2[q];
+                if (last) d[4 * QF * LS + 64] = f2[QF];
+            }
+            return;
+        }
+        const uint32_t o1 = 4u * (uint32_t)min(max(v1, 0), W - 1);
+        const uint32_t o2 = 4u * (uint32_t)min(max(v2, 0), W - 1);
+        float f1[RPW], f2[RPW];
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = min(wv + 4 * q, LR - 1);  // repeated last row: same values
+            const int u = y0 - a + r;
+            const uint32_t rb = 4u * (uint32_t)(min(max(u, 0), H - 1) * W);
+            f1[q] = *(const GLOBAL float*)(Pb + (rb + o1));
+            if (h2) f2[q] = *(const GLOBAL float*)(Pb + (rb + o2));
+        }
+        const bool c1 = v1 >= 0 && v1 < W;
+        const bool c2 = v2 >= 0 && v2 < W;
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = min(wv + 4 * q, LR - 1);
+            const int u = y0 - a + r;
+            const bool ur = u >= 0 && u < H;  // wave-uniform
+            L[r * LS + lane] = (ur && c1) ? f1[q] : __uint_as_float(NEG_INF_BITS);
+            if (h2) L[r * LS + 64 + lane] = (ur && c2) ? f2[q] : __uint_as_float(NEG_INF_BITS);
+        }
+    };
+    stage_tile(true);
+    __syncthreads();
+
+    // the max filter works on the bit patterns (see sliding_max)
+    const int32_t* Li = (const int32_t*)L;
+    // centre values of this thread's 16 outputs, read before the overwrite
+    // (lanes past the tile read a column inside it; they hold no output)
+    const int c = tid & 63;
+    const int cc = min(c, TILE_W - 1);
+    const int kw = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int32_t center[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) center[m] = Li[(kw * N + m + a) * LS + (cc + a)];
+
+    // row pass: RM[r][c] = max(L[r][c .. c+KD-1]), written over L[r][0..47]
+    int32_t y[NRUN][N];
+#pragma unroll
+    for (int k2 = 0; k2 < NRUN; ++k2) {
+        const int run = tid + k2 * 256;
+        if (run < LR * RUNS) {
+            const int r = run / RUNS, c0 = (run % RUNS) * N;
+            int32_t x[NIN];
+            const int32_t* src = Li + r * LS + c0;
+#pragma unroll
+            for (int m = 0; m < NV4; ++m) {
+                const int4 q = *(const int4*)(src + 4 * m);
+                if (4 * m + 0 < NIN) x[4 * m + 0] = q.x;
+                if (4 * m + 1 < NIN) x[4 * m + 1] = q.y;
+                if (4 * m + 2 < NIN) x[4 * m + 2] = q.z;
+                if (4 * m + 3 < NIN) x[4 * m + 3] = q.w;
+            }
+            sliding_max<N, KD>(x, y[k2]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k2 = 0; k2 < NRUN; ++k2) {
+        const int run = tid + k2 * 256;
+        if (run < LR * RUNS) {
+            const int r = run / RUNS, c0 = (run % RUNS) * N;
+            int32_t* dst = (int32_t*)L + r * LS + c0;
+#pragma unroll
+            for (int m = 0; m < N / 4; ++m)
+                *(int4*)(dst + 4 * m) =
+                    make_int4(y[k2][4 * m], y[k2][4 * m + 1], y[k2][4 * m + 2], y[k2][4 * m + 3]);
+        }
+    }
+    __syncthreads();
+
+    // column pass + candidate test: lane = column, wave = 16-row run
+    const int v = x0 + c;
+    int32_t cm[N];
+    {
+        int32_t x[NIN];
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) x[m] = Li[(kw * N + m) * LS + cc];
+        sliding_max<N, KD>(x, cm);
+    }
+
+    // candidates: the ballots of the wave's 16 rows stay in scalar registers
+    // (bitwise tests: no short-circuit branches)
+    // (lane masks combined in scalar registers: one vector compare each)
+    const uint64_t colmask = __builtin_amdgcn_ballot_w64((c < TILE_W) & (v < W) &
+                                                         (v - dp.band >= 0) & (v + dp.band < W));
+    const bool diw = dp.diw != 0;
+    // rows m of the wave's run that can hold a candidate: band <= u0 + m <= H - 1 - band
+    const int u0 = y0 + kw * N;
+    const int rlo = max(dp.band - u0, 0), rhi = min(H - 1 - dp.band - u0, N - 1);
+    const uint32_t rowbits = rhi >= rlo ? ((2u << rhi) - (1u << rlo)) : 0u;
+    uint64_t bal[N];
+    uint32_t mycnt = 0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        const uint64_t ge = __builtin_amdgcn_ballot_w64(center[m] >= cm[m]);
+        const uint64_t nz = diw ? __builtin_amdgcn_ballot_w64(center[m] != 0) : ~0ull;
+        bal[m] = (rowbits >> m) & 1u ? (colmask & ge & nz) : 0ull;
+        mycnt = (c == m) ? (uint32_t)__popcll(bal[m]) : mycnt;
+    }
+    if (c < N) s_rowcnt[kw * N + c] = mycnt;
+
+    if (dp.has_pct) {
+        // values in the two histogram bins holding the order statistics go to
+        // the tile's list (any order); out-of-frame centres are -inf and
+        // fall outside every range
+        // The two order statistics have adjacent ranks, so no value lies
+        // strictly between their bins: one range from the start of the lower
+        // bin to the end of the upper one holds exactly the two bins.
+        uint32_t lb, lw, hb, hw;
+        pct_bin_range(b.selstate[f * 4 + 0], lb, lw);
+        pct_bin_range(b.selstate[f * 4 + 2], hb, hw);
+        const uint32_t base = lw ? lb : hb;
+        const uint32_t width = c < TILE_W ? (hw ? (hb + hw - base) : lw) : 0u;
+        uint32_t mbits = 0;
+#pragma unroll
+        for (int m = N - 1; m >= 0; --m) {
+            const uint32_t bits = (uint32_t)center[m];
+            mbits = 2u * mbits + ((bits - base < width) ? 1u : 0u);
+        }
+        if (__ballot(mbits != 0u)) {  // wave-uniform
+            const uint32_t n = (uint32_t)__popc(mbits);
+            const uint32_t inc = wave_incl_scan(n, c);
+            const uint32_t tot = __shfl(inc, 63, 64);
+            uint32_t base = 0;
+            if (c == 0) base = atomicAdd(&s_lcnt, tot);
+            base = __builtin_amdgcn_readfirstlane(base);
+            uint32_t j = base + inc - n;
+            GLOBAL uint32_t* list = (GLOBAL uint32_t*)b.list + (size_t)item * TILE_PX;
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                if (mbits & (1u << m)) list[j++] = (uint32_t)center[m];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t cnt = s_rowcnt[tid];
+        const uint32_t inc = wave_incl_scan(cnt, tid);
+        s_rowcnt[tid] = inc - cnt;
+        if (tid == 63) {
+            b.tile_count[item] = inc;
+            s_total = inc;
+            if (dp.has_pct) b.tile_lcount[item] = s_lcnt;
+        }
+    }
+    __syncthreads();
+    GLOBAL uint16_t* slots = (GLOBAL uint16_t*)b.slots + (size_t)item * TILE_PX;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        if (bal[m] == 0) continue;  // wave-uniform: most 64-pixel row pieces hold no candidate
+        const int r = kw * N + m;
+        if ((bal[m] >> c) & 1ull) {
+            const uint32_t rank =
+                s_rowcnt[r] + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[m] >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal[m], 0u));
+            slots[rank] = (uint16_t)(r * TILE_W + c);
+        }
+    }
+
+    if constexpr (RI > 0) {
+        const uint32_t total = s_total;
+        if (total == 0) return;  // workgroup-uniform
+        __syncthreads();         // column pass done with L; slots visible
+        stage_tile(false);       // columns 0..63 again hold P
+        __syncthreads();
+        walk_tile<RI, LR, LC, LS>(b, dp, L, slots, total, item, P, H, W, x0, y0, a, tid);
+    }
+}
+
 // Generic detect for window sizes without a specialisation: straightforward
 // window loops over the staged tile.
 __global__ __launch_bounds__(256) void k_detect_generic(Batch b, DetParams dp) {
@@ -1270,9 +1528,26 @@ static hipError_t launch_det_t(const Batch& b, const DetParams& dp, hipStream_t 
     return hipGetLastError();
 }
 
+template <int KD>
+static hipError_t launch_det_rows_t(const Batch& b, const DetParams& dp, hipStream_t s) {
+    switch (dp.walk_ri) {
+        case 0: hipLaunchKernelGGL((k_detect_rows<KD, 0>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 1: hipLaunchKernelGGL((k_detect_rows<KD, 1>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 2: hipLaunchKernelGGL((k_detect_rows<KD, 2>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 3: hipLaunchKernelGGL((k_detect_rows<KD, 3>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 4: hipLaunchKernelGGL((k_detect_rows<KD, 4>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 5: hipLaunchKernelGGL((k_detect_rows<KD, 5>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 6: hipLaunchKernelGGL((k_detect_rows<KD, 6>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 7: hipLaunchKernelGGL((k_detect_rows<KD, 7>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 static bool detect_specialised(int kd) {
     switch (kd) {
         case 4: case 5: case 7: case 8: case 9: case 11: case 12: case 14: case 15: case 16:
+        case 18: case 19: case 21:
             return true;
         default:
             return false;
@@ -1307,6 +1582,9 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
         case 14: return launch_det_t<14>(b, dp, s);
         case 15: return launch_det_t<15>(b, dp, s);
         case 16: return launch_det_t<16>(b, dp, s);
+        case 18: return launch_det_rows_t<18>(b, dp, s);
+        case 19: return launch_det_rows_t<19>(b, dp, s);
+        case 21: return launch_det_rows_t<21>(b, dp, s);
         default: break;
     }
     if (detect_needs_plane(g)) {
