@@ -48,6 +48,10 @@ CASES = {
     # the default separation of diameter 13 (14): dilation window 19, the
     # row-first detect kernel (windows wider than the column-first one's lanes)
     "d13_default_sep": dict(diameter=13),
+    # window 22 (diameter 15, default separation 16) with the fused walk, and
+    # window 26 (separation 19) with radius 8, which walks in refine
+    "d15_default_sep": dict(diameter=15),
+    "d17_sep19": dict(diameter=17, separation=19, percentile=64),
     "pct0": dict(diameter=9, separation=9, percentile=0),
     "pct100": dict(diameter=9, separation=9, percentile=100),
     # dilation halo (3) narrower than the disk radius (5): walks near tile

@@ -1547,7 +1547,7 @@ static hipError_t launch_det_rows_t(const Batch& b, const DetParams& dp, hipStre
 static bool detect_specialised(int kd) {
     switch (kd) {
         case 4: case 5: case 7: case 8: case 9: case 11: case 12: case 14: case 15: case 16:
-        case 18: case 19: case 21:
+        case 18: case 19: case 21: case 22: case 24: case 25: case 26:
             return true;
         default:
             return false;
@@ -1585,6 +1585,10 @@ hipError_t launch_detect(const Batch& b, const Geom& g, const Geom* d_geom, hipS
         case 18: return launch_det_rows_t<18>(b, dp, s);
         case 19: return launch_det_rows_t<19>(b, dp, s);
         case 21: return launch_det_rows_t<21>(b, dp, s);
+        case 22: return launch_det_rows_t<22>(b, dp, s);
+        case 24: return launch_det_rows_t<24>(b, dp, s);
+        case 25: return launch_det_rows_t<25>(b, dp, s);
+        case 26: return launch_det_rows_t<26>(b, dp, s);
         default: break;
     }
     if (detect_needs_plane(g)) {
