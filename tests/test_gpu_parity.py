@@ -48,9 +48,11 @@ CASES = {
     # the default separation of diameter 13 (14): dilation window 19, the
     # row-first detect kernel (windows wider than the column-first one's lanes)
     "d13_default_sep": dict(diameter=13),
-    # window 22 (diameter 15, default separation 16) with the fused walk, and
-    # window 26 (separation 19) with radius 8, which walks in refine
+    # window 22 (diameter 15, default separation 16), and windows 25 and 26
+    # with smoothing radius 8 and disk radius 8 (diameter 17), all with the
+    # walk fused into detect
     "d15_default_sep": dict(diameter=15),
+    "d17_default_sep": dict(diameter=17),
     "d17_sep19": dict(diameter=17, separation=19, percentile=64),
     "pct0": dict(diameter=9, separation=9, percentile=0),
     "pct100": dict(diameter=9, separation=9, percentile=100),

@@ -219,7 +219,7 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 #endif
 
 struct K1Params {
-    float w[15];
+    float w[17];
     float gn, rgn, kk, rkk;
     int seg;
     int hist;
@@ -504,7 +504,7 @@ void k_bandpass(Batch b, K1Params kp) {
     }
 }
 
-// Generic bandpass for R > 7: two straightforward passes through a global
+// Generic bandpass for R > 8 (and float input or unaligned rows): two straightforward passes through a global
 // temporary, each thread one pixel, taps in the shader's order.
 __global__ void k_bandpass_rows_generic(Batch b, const Geom* __restrict__ g) {
     const uint32_t f = blockIdx.y;
@@ -584,12 +584,13 @@ static hipError_t launch_bp_r(int R, const Batch& b, const K1Params& kp, bool fa
         case 5: return launch_bp_t<5, T>(b, kp, fast, s);
         case 6: return launch_bp_t<6, T>(b, kp, fast, s);
         case 7: return launch_bp_t<7, T>(b, kp, fast, s);
+        case 8: return launch_bp_t<8, T>(b, kp, fast, s);
         default: return hipErrorInvalidValue;
     }
 }
 
 bool bandpass_fast_path(const Batch& b, const Geom& g) {
-    return g.R >= 1 && g.R <= 7 && b.vec && (b.in_type == IN_U16 || b.in_type == IN_U8);
+    return g.R >= 1 && g.R <= 8 && b.vec && (b.in_type == IN_U16 || b.in_type == IN_U8);
 }
 
 hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, int seg_rows,
@@ -597,7 +598,7 @@ hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, in
     (void)d_geom;
     if (!bandpass_fast_path(b, g)) return hipErrorInvalidValue;
     K1Params kp;
-    for (int i = 0; i < 15; ++i) kp.w[i] = (i < 2 * g.R + 1) ? g.w[i] : 0.f;
+    for (int i = 0; i < 17; ++i) kp.w[i] = (i < 2 * g.R + 1) ? g.w[i] : 0.f;
     kp.gn = g.gn; kp.rgn = g.rgn; kp.kk = g.kk; kp.rkk = g.rkk;
     kp.seg = seg_rows;
     kp.hist = b.hist1 != nullptr && g.has_pct;
@@ -1539,6 +1540,13 @@ static hipError_t launch_det_rows_t(const Batch& b, const DetParams& dp, hipStre
         case 5: hipLaunchKernelGGL((k_detect_rows<KD, 5>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
         case 6: hipLaunchKernelGGL((k_detect_rows<KD, 6>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
         case 7: hipLaunchKernelGGL((k_detect_rows<KD, 7>), dim3(b.tiles), dim3(256), 0, s, b, dp); break;
+        case 8:  // disk radius 8 (diameters 17, 18) walks here from window 22 on
+            if constexpr (KD >= 22) {
+                hipLaunchKernelGGL((k_detect_rows<KD, 8>), dim3(b.tiles), dim3(256), 0, s, b, dp);
+                break;
+            } else {
+                return hipErrorInvalidValue;
+            }
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1555,7 +1563,8 @@ static bool detect_specialised(int kd) {
 }
 
 bool walk_in_detect_supported(const Geom& g) {
-    return detect_specialised(g.kd) && g.rint >= 1 && g.rint <= 7 && g.max_iterations >= 1 &&
+    const int max_ri = g.kd >= 22 ? 8 : 7;
+    return detect_specialised(g.kd) && g.rint >= 1 && g.rint <= max_ri && g.max_iterations >= 1 &&
            !g.characterize && !g.cpu_processed;
 }
 
