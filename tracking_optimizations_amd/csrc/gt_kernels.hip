@@ -1039,8 +1039,8 @@ __global__ __launch_bounds__(256) void k_detect(Batch b, DetParams dp) {
     }
 }
 
-// Detect for dilation windows too wide for the lanes of k_detect (KD > 16,
-// separation 13 and more), row first: one workgroup per 48x64 tile, the tile
+// Detect for dilation windows too wide for the lanes of k_detect (KD 18 to
+// 26, separation 13 to 19), row first: one workgroup per 48x64 tile, the tile
 // and its window halo staged in LDS (all loads issued before the first
 // store); the row maxima overwrite the staged tile in place, so one tile
 // needs (64+KD-1) x (48+KD+3) floats of LDS and six workgroups share a CU.
