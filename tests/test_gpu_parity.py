@@ -130,6 +130,46 @@ def test_float32_input(ctx, frames512, case):
     _check(rows, cand, ref_rows, ref_cand, f"f32/{case}")
 
 
+def _float_frames(frames, seed):
+    """float32 frames whose values carry full 24-bit mantissas and some
+    negative values: boxcar sums of them round differently when taken in
+    another order (checked here on the CPU)."""
+    rng = np.random.default_rng(seed)
+    f = (frames.astype(np.float32) * np.float32(0.7310585)
+         + rng.random(frames.shape, dtype=np.float32) * np.float32(3.0) - np.float32(1.0))
+    f[:, ::37, :] *= np.float32(-0.5)   # rows of negative values
+    # a running boxcar (add the entering tap, drop the leaving one) differs
+    # from the tap-by-tap sum on some pixels of these frames
+    row = f[0, 7]
+    k = 11
+    direct = [np.float32(0.0)] * (len(row) - k + 1)
+    for j in range(len(direct)):
+        acc = np.float32(0.0)
+        for i in range(k):
+            acc = np.float32(acc + row[j + i])
+        direct[j] = acc
+    run = [direct[0]]
+    for j in range(1, len(direct)):
+        run.append(np.float32(np.float32(run[-1] + row[j + k - 1]) - row[j - 1]))
+    assert any(a != b for a, b in zip(direct, run))
+    return np.ascontiguousarray(f)
+
+
+@pytest.mark.parametrize("case", ["c1", "defaults", "wide_smooth", "d13_default_sep", "iter1"])
+def test_float32_rounding(ctx, frames512, case):
+    """float32 input with rounding in its boxcar sums, on the streaming
+    bandpass (widths that are multiples of 4: 512, and 100 with strips at
+    both frame edges) and the generic one (width 98)."""
+    kw = CASES[case]
+    f = _float_frames(frames512[:2], 5)
+    for sub in (f, f[:, :100, :100], f[:, :90, :98]):
+        sub = np.ascontiguousarray(sub)
+        print(f"[parity] float32 rounding case={case} shape={sub.shape}", flush=True)
+        rows, cand, _, _ = locate(sub, make_params(**kw), ctx=ctx)
+        ref_rows, ref_cand = _oracle_movie(sub, kw)
+        _check(rows, cand, ref_rows, ref_cand, f"f32r/{case}/{sub.shape}")
+
+
 @pytest.mark.parametrize("case", ["c1", "defaults", "char", "small_sep"])
 def test_golden_grey_lion_u8(ctx, grey_lion, case):
     kw = CASES[case]

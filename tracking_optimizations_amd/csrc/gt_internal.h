@@ -145,7 +145,7 @@ struct AdjOut {
 // Launchers (gt_kernels.hip).  All launch on `s`; return hipSuccess or the
 // first launch error.
 hipError_t launch_zero_batch(const Batch& b, hipStream_t s);
-// The streaming bandpass handles integer input with 1 <= R <= 8, W % 4 == 0
+// The streaming bandpass handles uint8, uint16 and float32 input with 1 <= R <= 8, W % 4 == 0
 // and aligned frame bases (b.vec); everything else takes the generic kernels.
 bool bandpass_fast_path(const Batch& b, const Geom& g);
 hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom,

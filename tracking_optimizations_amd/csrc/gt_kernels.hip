@@ -228,6 +228,7 @@ struct K1Params {
 template <typename T> struct Chunk4;
 template <> struct Chunk4<uint16_t> { uint2 v; };
 template <> struct Chunk4<uint8_t> { uint32_t v; };
+template <> struct Chunk4<float> { float4 v; };
 
 // base: the frame (wave-uniform); off: byte offset, 32-bit (frames hold
 // fewer than 2^30 pixels), so the load takes the frame as a scalar base
@@ -235,7 +236,8 @@ template <typename T>
 __device__ __forceinline__ Chunk4<T> ld_chunk(const GLOBAL char* base, uint32_t off) {
     const GLOBAL char* p = base + off;
     Chunk4<T> c;
-    if constexpr (sizeof(T) == 2) c.v = *(const GLOBAL uint2*)p;
+    if constexpr (IsFloat<T>::v) c.v = *(const GLOBAL float4*)p;
+    else if constexpr (sizeof(T) == 2) c.v = *(const GLOBAL uint2*)p;
     else c.v = *(const GLOBAL uint32_t*)p;
     return c;
 }
@@ -245,7 +247,9 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 template <typename T>
 __device__ __forceinline__ f4v unpack_chunk(const Chunk4<T>& c) {
     f4v r;
-    if constexpr (sizeof(T) == 2) {
+    if constexpr (IsFloat<T>::v) {
+        r.x = c.v.x; r.y = c.v.y; r.z = c.v.z; r.w = c.v.w;
+    } else if constexpr (sizeof(T) == 2) {
         r.x = (float)(c.v.x & 0xffffu); r.y = (float)(c.v.x >> 16);
         r.z = (float)(c.v.y & 0xffffu); r.w = (float)(c.v.y >> 16);
     } else {
@@ -302,6 +306,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
 void k_bandpass(Batch b, K1Params kp) {
     constexpr int NT = 2 * R + 1;
     static_assert(R <= 8, "taps must stay inside the staged halo");
+    // float input: the boxcar sums are not exact, so they are formed tap by
+    // tap in the shader's order (from +0, as preprocess_*.wgsl do) instead
+    // of as running sums, and the clamp keeps NaN as the shader's select does
+    constexpr bool FLT = IsFloat<T>::v;
     // per parity: boxcar copy (mirrored) then Gaussian copy (zero outside)
     __shared__ __attribute__((aligned(16))) float buf[2][2][K1_ROWBUF];
     // two 16-bit bins per word (word i: bins 2i + 1 and 2i + 2; zeros are
@@ -358,7 +366,9 @@ void k_bandpass(Batch b, K1Params kp) {
 
     // prefetch: row vr is loaded PF rows ahead into slot (vr - vr0) % NT,
     // so only PF + 1 slots are live at a time
-    constexpr int PF = NT < 4 ? NT : 4;
+    // (float chunks take 4 registers each: 3 rows ahead)
+    constexpr int PFMAX = FLT ? 3 : 4;
+    constexpr int PF = NT < PFMAX ? NT : PFMAX;
     Chunk4<T> pm[NT];
 #pragma unroll
     for (int s = 0; s < PF; ++s) {
@@ -405,7 +415,15 @@ void k_bandpass(Batch b, K1Params kp) {
                 float v[20];
                 lds_ld20(bb_ + rbase, v);
                 float tb[4];
-                {
+                if constexpr (FLT) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        float bx = 0.f;
+#pragma unroll
+                        for (int i = -R; i <= R; ++i) bx = bx + v[8 + q + i];
+                        tb[q] = bx;
+                    }
+                } else {
                     float bx = v[8 - R];
 #pragma unroll
                     for (int i = -R + 1; i <= R; ++i) bx = bx + v[8 + i];
@@ -442,7 +460,7 @@ void k_bandpass(Batch b, K1Params kp) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const float gq = tg[q];
-                    vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
+                    if constexpr (!FLT) vbox[q] = (vbox[q] - rb[s][q]) + tb[q];
                     rg[s][q] = gq;
                     rb[s][q] = tb[q];
                 }
@@ -456,9 +474,18 @@ void k_bandpass(Batch b, K1Params kp) {
 #pragma unroll
                         for (int j = 0; j < NT; ++j)
                             gsum = __builtin_fmaf(w[j], rg[(s + 1 + j) % NT][q], gsum);
-                        const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
-                                          cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
-                        out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
+                        if constexpr (FLT) {
+                            float bsum = 0.f;
+#pragma unroll
+                            for (int j = 0; j < NT; ++j) bsum = bsum + rb[(s + 1 + j) % NT][q];
+                            const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
+                                              cdiv_t(bsum, kp.kk, kp.rkk, FAST);
+                            out[q] = res < 0.f ? 0.f : res;
+                        } else {
+                            const float res = cdiv_t(gsum, kp.gn, kp.rgn, FAST) -
+                                              cdiv_t(vbox[q], kp.kk, kp.rkk, FAST);
+                            out[q] = __builtin_fmaxf(res, 0.f);   // res is finite, never -0
+                        }
                     }
                     if (outl) {
                         // the row offset goes through a scalar register, so
@@ -477,9 +504,12 @@ void k_bandpass(Batch b, K1Params kp) {
                         // both) instead of contending for one word
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
-                            const uint32_t bits = __float_as_uint(out[q]);
+                            uint32_t bits = __float_as_uint(out[q]);
                             const bool pos = bits != 0u;
                             zc += (uint32_t)__popcll(~__builtin_amdgcn_ballot_w64(pos) & outl_m);
+                            // float input can give +inf or NaN: the last bin
+                            // takes them (bin 4095: word 2047, low half)
+                            if constexpr (FLT) bits = min(bits, 0x7ff7ffffu);
                             if (outl && pos)
                                 atomicAdd(&s_hist[bits >> 20], 1u << ((bits >> 15) & 16u));
                         }
@@ -504,7 +534,7 @@ void k_bandpass(Batch b, K1Params kp) {
     }
 }
 
-// Generic bandpass for R > 8 (and float input or unaligned rows): two straightforward passes through a global
+// Generic bandpass for R > 8 (and rows whose width is not a multiple of 4): two straightforward passes through a global
 // temporary, each thread one pixel, taps in the shader's order.
 __global__ void k_bandpass_rows_generic(Batch b, const Geom* __restrict__ g) {
     const uint32_t f = blockIdx.y;
@@ -590,7 +620,8 @@ static hipError_t launch_bp_r(int R, const Batch& b, const K1Params& kp, bool fa
 }
 
 bool bandpass_fast_path(const Batch& b, const Geom& g) {
-    return g.R >= 1 && g.R <= 8 && b.vec && (b.in_type == IN_U16 || b.in_type == IN_U8);
+    return g.R >= 1 && g.R <= 8 && b.vec &&
+           (b.in_type == IN_U16 || b.in_type == IN_U8 || b.in_type == IN_F32);
 }
 
 hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, int seg_rows,
@@ -606,6 +637,7 @@ hipError_t launch_bandpass(const Batch& b, const Geom& g, const Geom* d_geom, in
     if (kp.hist && (int64_t)K1_STRIP * seg_rows >= 65536) return hipErrorInvalidValue;
     const bool fast = g.fastdiv_gn && g.fastdiv_kk;
     if (b.in_type == IN_U16) return launch_bp_r<uint16_t>(g.R, b, kp, fast, s);
+    if (b.in_type == IN_F32) return launch_bp_r<float>(g.R, b, kp, fast, s);
     return launch_bp_r<uint8_t>(g.R, b, kp, fast, s);
 }
 
