@@ -43,12 +43,13 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(h, w, n_cand, n_rows, ncols, rint):
+def algorithmic_bytes(h, w, n_cand, n_rows, ncols, rint, in_bytes=2):
     """Bytes of the three-kernel locate pipeline per frame, each counted once:
-    u16 frame read (2), float32 bandpass written (4) and read once (4) per
-    pixel; per candidate its index (4) and one (2r+1)^2 box read; 4 bytes per
-    output value."""
-    return 10.0 * h * w + n_cand * (4.0 + 4.0 * (2 * rint + 1) ** 2) + 4.0 * ncols * n_rows
+    frame read (in_bytes: 2 for uint16), float32 bandpass written (4) and
+    read once (4) per pixel; per candidate its index (4) and one (2r+1)^2 box
+    read; 4 bytes per output value."""
+    return ((in_bytes + 8.0) * h * w + n_cand * (4.0 + 4.0 * (2 * rint + 1) ** 2)
+            + 4.0 * ncols * n_rows)
 
 
 _NATIVE_ORACLE = None
@@ -237,6 +238,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-size "
                          "kernel for scaling hardware byte counters)")
+    ap.add_argument("--input", choices=["u16", "f32"], default="u16",
+                    help="frame type of the movie workload: uint16, or float32 frames holding "
+                         "the same values (the reference's own frame type)")
     ap.add_argument("--workload", choices=["movie", "uneven", "adjoint", "link", "file", "raw"],
                     default="movie",
                     help="movie: 1000 x 2048^2 (headline); uneven: 4096 mixed-size frames; "
@@ -289,11 +293,24 @@ def main() -> int:
     movie = synth.generate(ctx, spec)
     ctx.synchronize()
     log(f"movie ready in {time.perf_counter() - t0:.1f} s ({movie.nbytes / 1e9:.2f} GB)")
+    locate_fn, frames_ptr = L.gt_locate_u16, movie.ptr
+    if args.input == "f32":
+        # float32 frames holding the uint16 values, converted on the host in
+        # chunks of 32 frames
+        fmovie = synth.DeviceBuffer(ctx, nf * H * W * 4)
+        for f0 in range(0, nf, 32):
+            k = min(32, nf - f0)
+            chunk = movie.to_host(np.uint16, (k, H, W), f0 * H * W * 2).astype(np.float32)
+            _lib.check(L.gt_memcpy_h2d(ctx.handle, ctypes.c_void_p(fmovie.ptr + f0 * H * W * 4),
+                                       chunk.ctypes.data_as(ctypes.c_void_p), chunk.nbytes))
+        ctx.synchronize()
+        locate_fn, frames_ptr = L.gt_locate_f32, fmovie.ptr
+        log(f"float32 copy of the movie ready ({fmovie.nbytes / 1e9:.2f} GB)")
 
     def run(flags):
         out = ctypes.c_void_p()
-        _lib.check(L.gt_locate_u16(ctx.handle, ctypes.c_void_p(movie.ptr), nf, H, W,
-                                   ctypes.byref(params), flags, ctypes.byref(out)))
+        _lib.check(locate_fn(ctx.handle, ctypes.c_void_p(frames_ptr), nf, H, W,
+                             ctypes.byref(params), flags, ctypes.byref(out)))
         r = _lib.Result(out)
         t = ctx.timing()
         del r
@@ -380,9 +397,10 @@ def main() -> int:
     dom_ms, dom_n = stages[dominant]
     rint = (kw["diameter"] - 1) // 2
     px = float(H) * W * nf
+    in_b = 2 if args.input == "u16" else 4
     # algorithmic bytes of each kernel over the whole step
     kernel_bytes = {
-        "bandpass": px * (2 + 4),                       # u16 in, float32 out
+        "bandpass": px * (in_b + 4),                    # frame in, float32 out
         # float32 in, candidate slots out, plus (centroid walk fused in
         # detect) one disk box read and one record written per candidate
         "detect": px * 4 + n_cand * (2 + 4 * (2 * rint + 1) ** 2 + 16),
@@ -398,12 +416,12 @@ def main() -> int:
     # profiles/traffic_movie_2048.json) for this launch's frame count
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_movie_2048.json")
-    if H == 2048 and W == 2048 and os.path.exists(tpath):
+    if H == 2048 and W == 2048 and args.input == "u16" and os.path.exists(tpath):
         tk = json.load(open(tpath)).get("kernels", {})
         for name, rec in tk.items():
             if name.startswith(f"gt::k_{dominant}") and rec.get("read_bytes_per_frame") is not None:
                 traffic = (rec["read_bytes_per_frame"] + rec["write_bytes_per_frame"]) * nf / max(dom_n, 1)
-    pipe_bytes = algorithmic_bytes(H, W, n_cand / nf, n_rows / nf, 4, rint) * nf
+    pipe_bytes = algorithmic_bytes(H, W, n_cand / nf, n_rows / nf, 4, rint, in_b) * nf
     pipe_gbs = pipe_bytes / (elapsed / args.steps) / 1e9
     log(f"dominant kernel {dominant}: {achieved:.0f} GB/s over {per_launch_bytes / 1e6:.1f} MB "
         f"per launch; pipeline {pipe_gbs:.0f} GB/s of algorithmic bytes "
@@ -479,7 +497,9 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded device generator), device-resident",
-        "config": {"workload": f"{nf} frames {H}x{W} uint16, 5000 spots/frame, sigma 2, "
+        "config": {"workload": f"{nf} frames {H}x{W} "
+                               f"{'uint16' if args.input == 'u16' else 'float32 (uint16 values)'}, "
+                               "5000 spots/frame, sigma 2, "
                                "Brownian D=0.5, bg 150 + Poisson",
                    "params": kw, "path": path, "frames": nf},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved,
