@@ -371,9 +371,7 @@ void k_bandpass(Batch b, K1Params kp) {
     constexpr int PF = NT < PFMAX ? NT : PFMAX;
     Chunk4<T> pm[NT];
 #pragma unroll
-    for (int s = 0; s < PF; ++s) {
-        if (vr0 + s < vr1) pm[s] = ld_chunk<T>(inb, row_off(vr0 + s) + cboff);
-    }
+    for (int s = 0; s < PF; ++s) pm[s] = ld_chunk<T>(inb, row_off(min(vr0 + s, vr1 - 1)) + cboff);
 
     uint32_t zc = 0;  // zero outputs of this work item (wave-uniform)
     const uint64_t outl_m = __builtin_amdgcn_ballot_w64(outl);
@@ -382,11 +380,15 @@ void k_bandpass(Batch b, K1Params kp) {
     auto rows = [&](auto border_c) {
         constexpr bool BORDER = decltype(border_c)::value;
         int parity = 0;
+        // whole groups of NT rows: the last group runs past vr1 (those rows
+        // load row vr1 - 1 and emit nothing) so that no branch surrounds a
+        // row; one there makes the compiler wait for every load and store in
+        // flight before each row instead of only for the row's own chunk
         for (int base = vr0; base < vr1; base += NT) {
 #pragma unroll
             for (int s = 0; s < NT; ++s) {
                 const int vr = base + s;
-                if (vr < vr1) {  // wave-uniform
+                {
                 float* bb_ = buf[parity][0];
                 float* bg_ = buf[parity][1];
                 parity ^= 1;
@@ -408,7 +410,7 @@ void k_bandpass(Batch b, K1Params kp) {
                         *(f4v*)(bb_ + 4 * t) = m;
                     }
                 }
-                if (vr + PF < vr1) pm[(s + PF) % NT] = ld_chunk<T>(inb, row_off(vr + PF) + cboff);
+                pm[(s + PF) % NT] = ld_chunk<T>(inb, row_off(min(vr + PF, vr1 - 1)) + cboff);
                 __syncthreads();
 
                 // boxcar row pass on the mirrored copy (integer sums: exact)
@@ -465,7 +467,7 @@ void k_bandpass(Batch b, K1Params kp) {
                     rb[s][q] = tb[q];
                 }
 
-                if (vr >= y0 + R) {
+                if (vr >= y0 + R && vr < vr1) {  // wave-uniform
                     const int o = vr - R;
                     float out[4];
 #pragma unroll
@@ -515,7 +517,7 @@ void k_bandpass(Batch b, K1Params kp) {
                         }
                     }
                 }
-                }  // vr < vr1
+                }
             }
         }
     };
