@@ -170,6 +170,20 @@ def test_float32_rounding(ctx, frames512, case):
         _check(rows, cand, ref_rows, ref_cand, f"f32r/{case}/{sub.shape}")
 
 
+@pytest.mark.parametrize("scale", [1e-33, 1e-25, 1e30, 1e33])
+def test_float32_extreme_magnitudes(ctx, frames512, scale):
+    """float32 frames of very small and very large magnitudes (sums from
+    about 1e-31 to 1e38): the float32 bandpass keeps the shader's exact
+    divisions and sums at every scale."""
+    kw = dict(diameter=9, separation=9, minmass=0.0, percentile=64)
+    f = np.ascontiguousarray((_float_frames(frames512[:1], 7) * np.float32(scale)).astype(np.float32))
+    f = np.ascontiguousarray(f[:, :200, :256])
+    print(f"[parity] float32 extreme magnitudes scale={scale}", flush=True)
+    rows, cand, _, _ = locate(f, make_params(**kw), ctx=ctx)
+    ref_rows, ref_cand = _oracle_movie(f, kw)
+    _check(rows, cand, ref_rows, ref_cand, f"f32x/{scale}")
+
+
 @pytest.mark.parametrize("case", ["c1", "defaults", "char", "small_sep"])
 def test_golden_grey_lion_u8(ctx, grey_lion, case):
     kw = CASES[case]
